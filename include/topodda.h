@@ -1,0 +1,146 @@
+/*
+ * topodda.h -- C ABI of the gfx950 DDA solve path.
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved (re, im)
+ * doubles.  Every function returns 0 on success or a TDDA_ERR_* code;
+ * tdda_last_error() gives the message of the last failure on this thread.
+ *
+ * The model entry points replace the reference's pybind11 DDAModel
+ * (python_bindings/PythonBindings.cpp:16-70); each declaration cites the
+ * reference member it stands for.
+ */
+#ifndef TOPODDA_H_
+#define TOPODDA_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDDA_OK 0
+#define TDDA_ERR_INVALID 1  /* reference: `throw 1` (pybind RuntimeError) */
+#define TDDA_ERR_METHOD 2   /* reference: std::invalid_argument (pybind ValueError) */
+#define TDDA_ERR_HIP 3      /* HIP runtime failure */
+#define TDDA_ERR_MEMORY 4   /* device memory exhausted / over budget */
+#define TDDA_ERR_STATE 5    /* call order violated */
+
+#define TDDA_PREC_FP64 0          /* complex128 transforms and Krylov vectors */
+#define TDDA_PREC_FP32_REFINED 1  /* complex64 transforms/Krylov, FP64 reliable updates */
+
+#define TDDA_GRAD_REFERENCE 0  /* one-sided FD partials as DDAModel.cpp:471-578 */
+#define TDDA_GRAD_EXACT 1      /* closed-form partials, same adjoint solve */
+
+#define TDDA_STATUS_ACTIVE 0
+#define TDDA_STATUS_CONVERGED 1
+#define TDDA_STATUS_BREAKDOWN 2
+#define TDDA_STATUS_MAXIT 3
+
+typedef struct tdda_model tdda_model;
+
+/* Constructor arguments of DDAModel, in the order and meaning of
+ * python_bindings/PythonBindings.cpp:19-59 (src/DDAModel.cpp:79). */
+typedef struct {
+    double betaMin, betaMax, ita;
+    const char* betaType;
+    const int* filterIterations;
+    long nFilterIterations;
+    const double* filterRadii;
+    long nFilterRadii;
+    int useFilter;
+    const char* symmetryType; /* "None" | "4fold" */
+    const double* symmetryAxis;
+    long nSymmetryAxis;
+    int isPeriodic;
+    const char* objectiveName;
+    const double* objPara;
+    long nObjPara;
+    const int* geometry; /* 3*num_total */
+    long nGeometry;
+    const double* inputDielectrics; /* 3*num_total */
+    long nInputDielectrics;
+    int num_x, num_y, num_z, num_total;
+    const double* n_K; /* 3 */
+    double E0;
+    const double* n_E0; /* 3 */
+    double lam;
+    const double* material; /* nMaterial complex, interleaved */
+    long nMaterial;
+    double nback;
+    int MAXm, MAXn;
+    double Lm, Ln;                  /* image periods in cells */
+    const char* matrixAMethodName;  /* "FCD" | "LDR" */
+    double d;
+    const double* sineIntegralValues;
+    long nSine;
+    const double* cosineIntegralValues;
+    long nCosine;
+    double integralDelta;
+    int verbose;
+} tdda_model_args;
+
+typedef struct {
+    int status;      /* TDDA_STATUS_* */
+    int iterations;  /* the reference's ITERATION */
+    double error;    /* final ||r|| / ||rhs|| (recursive residual in FP64 mode, true residual in refined mode) */
+    int reliable_updates;
+    double ms;       /* wall time of the solve */
+} tdda_solve_info;
+
+/* last failure message of this thread ("" if none) */
+const char* tdda_last_error(void);
+
+/* DDAModel::DDAModel (src/DDAModel.cpp:79-333), PythonBindings.cpp:19-59 */
+int tdda_model_create(const tdda_model_args* args, tdda_model** out);
+/* DDAModel::~DDAModel (src/DDAModel.cpp:335-338) */
+int tdda_model_destroy(tdda_model* m);
+/* DDAModel::solveElectricField (src/DDAModel.cpp:874-885), PythonBindings.cpp:64;
+ * P0: n complex (n = 3*num_total) */
+int tdda_model_solve_electric_field(tdda_model* m, const double* P0, long n, int maxit, double tol);
+/* DDAModel::calculateObjective (src/DDAModel.cpp:870-872), PythonBindings.cpp:61 */
+int tdda_model_calculate_objective(tdda_model* m, double* value);
+/* DDAModel::calculateGradients (src/DDAModel.cpp:398-469), PythonBindings.cpp:62;
+ * grad: n = NFpara doubles */
+int tdda_model_calculate_gradients(tdda_model* m, double epsilon_partial, double original_obj, int maxit,
+                                   double tol, double* grad, long n);
+/* DDAModel::UpdateParameters (src/DDAModel.cpp:747-772), PythonBindings.cpp:66 */
+int tdda_model_set_parameters(tdda_model* m, const double* values, long n);
+/* DDAModel::get_parameter_copy (src/DDAModel.cpp:1341-1343), PythonBindings.cpp:67 */
+int tdda_model_get_parameters(tdda_model* m, double* out, long n);
+/* DDAModel::get_dielectrics_copy (src/DDAModel.cpp:1346-1348), PythonBindings.cpp:68; n = 3*num_total */
+int tdda_model_get_dielectrics(tdda_model* m, double* out, long n);
+/* DDAModel::get_EResult (src/DDAModel.cpp:1254-1256), PythonBindings.cpp:69; n = 3*num_total complex */
+int tdda_model_get_electric_field(tdda_model* m, double* out, long n);
+
+/* Sizes for the getters above (NFpara, num_total). */
+int tdda_model_sizes(tdda_model* m, long* nfpara, long* ndipoles);
+/* Solver precision for later solves (TDDA_PREC_*). No reference counterpart: the reference is FP64 only. */
+int tdda_model_set_precision(tdda_model* m, int precision);
+/* Gradient partials mode (TDDA_GRAD_*). No reference counterpart. */
+int tdda_model_set_gradient_mode(tdda_model* m, int mode);
+/* Iteration count / residual of the last forward (which=0) or adjoint (which=1) solve;
+ * the reference keeps ITERATION and Error unbound (src/DDAModel.h:64-66). */
+int tdda_model_solve_info(tdda_model* m, int which, tdda_solve_info* out);
+/* Current polarisation P (n = 3*num_total complex) and 1/alpha per dipole (n = num_total complex). */
+int tdda_model_get_polarization(tdda_model* m, double* out, long n);
+int tdda_model_get_inverse_alpha(tdda_model* m, double* out, long n);
+/* y = (diag(al) + A) x on the device in FP64 (DDAModel::Aproductwithalb, src/DDAModel.cpp:1194-1203). */
+int tdda_model_apply(tdda_model* m, const double* x, double* y, long n);
+/* FFT grid chosen for the model (Lx, Ly, Lz) and Green's setup time in ms. */
+int tdda_model_grid(tdda_model* m, int* lxyz, double* setup_ms);
+
+/* Re-solve the forward system from P = 0 with the right-hand side and 1/alpha
+ * already on the device and no copy back (throughput timing of
+ * DDAModel::bicgstab, src/DDAModel.cpp:899-1009). */
+int tdda_model_solve_resident(tdda_model* m, int maxit, double tol, tdda_solve_info* out);
+/* Average ms per launch of the five matvec passes (X fwd, Y fwd, Z, Y inv,
+ * X inv) at the current precision, timed with HIP events on the library stream;
+ * the matvec is AProductCore::Aproduct + al*b (src/AProductCore.cpp:461-515). */
+int tdda_model_time_passes(tdda_model* m, int reps, double* ms5);
+
+/* Bytes of device memory currently held by the library. */
+long tdda_device_bytes(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPODDA_H_ */

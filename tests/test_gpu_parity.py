@@ -1,0 +1,218 @@
+"""GPU parity of the compat DDAModel against the FP64 oracle (small sizes).
+
+Tolerances: operator application in FP64 to 1e-11 relative; solves compared
+against the oracle's solve with the same algorithm and tolerance, and against
+a tightly converged oracle solution; gradients to the stated bounds below.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, GOLDEN_OBJECTIVE
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dm():
+    import dda_model
+    return dda_model
+
+
+@pytest.fixture(scope="module")
+def O():
+    import dda_oracle
+    return dda_oracle
+
+
+def _args(O, tables, shape, method="LDR", symmetry="None", maxmn=0, seed=0, lam=542.0, d=15.0,
+          mat=(1.0 + 0j, 5.96 + 3.89j), k=(0, 0, 1), e=(1, 0, 0), geometry=None, diel=None):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    if diel is None:
+        diel = np.repeat(rng.uniform(0.0, 1.0, n), 3)
+    a = O.wrapper_args(list(shape), d, diel, list(mat), list(k), list(e), lam,
+                       symmetry_axes=[(nx - 1) / 2, (ny - 1) / 2], symmetry_method=symmetry,
+                       apc_max_m=maxmn, apc_max_n=maxmn, apc_method_name=method, tables=tables)
+    if geometry is not None:
+        a[12] = geometry
+        a[17] = geometry.size // 3
+    return a
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+CASES = [
+    ("ldr-5x4x3", dict(shape=(5, 4, 3), method="LDR")),
+    ("ldr-16cube", dict(shape=(16, 16, 16), method="LDR")),
+    ("fcd-periodic-6x5x4", dict(shape=(6, 5, 4), method="FCD", maxmn=3)),
+    ("ldr-7x9x11-odd", dict(shape=(7, 9, 11), method="LDR", k=(0.6, 0, 0.8), e=(0.8, 0, -0.6))),
+    ("ldr-1x1x1", dict(shape=(1, 1, 1), method="LDR")),
+    ("ldr-24x3x2", dict(shape=(24, 3, 2), method="LDR")),
+]
+
+
+@pytest.mark.parametrize("name,case", CASES, ids=[c[0] for c in CASES])
+def test_operator_apply_matches_oracle(dm, O, tables, name, case):
+    _log(f"[gpu] operator apply {name}")
+    a = _args(O, tables, **case)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    rng = np.random.default_rng(7)
+    n = 3 * o.N
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = g.applyOperator(x)
+    assert _rel(y, o.apply(x)) < 1e-11
+    assert np.allclose(g.getInverseAlpha(), o.al[0::3], rtol=1e-13)
+
+
+def test_operator_apply_scattered_geometry(dm, O, tables):
+    """Dipoles on a subset of the box in shuffled order (the box map path)."""
+    _log("[gpu] operator apply scattered geometry")
+    shape = (6, 5, 4)
+    full = O.generate_geometry(*shape).reshape(-1, 3)
+    rng = np.random.default_rng(11)
+    pick = np.sort(rng.permutation(len(full))[:70])
+    pick = rng.permutation(pick)
+    geo = full[pick].reshape(-1)
+    diel = np.repeat(rng.uniform(0, 1, 70), 3)
+    a = _args(O, tables, shape, geometry=geo, diel=diel)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    x = rng.standard_normal(210) + 1j * rng.standard_normal(210)
+    assert _rel(g.applyOperator(x), o.apply(x)) < 1e-11
+
+
+def test_golden_wrapper_objective(dm, golden_inputs):
+    """python_tests/test_DDAModelWrapper.py:32-40 on the GPU path."""
+    _log("[gpu] golden wrapper objective (22x22x10 FCD +-50)")
+    diel, _ = golden_inputs
+    model = dm.DDAModelWrapper([22, 22, 10], 15.0, diel, dielectric_constants=[1.01 + 0j, 5.96282 + 3.80423e-7j],
+                               light_direction=[0, 0, 1], light_polarization=[1, 0, 0], light_wavelength_nm=542,
+                               symmetry_axes=[10.5, 10.5], verbose=False)
+    obj = model.objective()
+    _log(f"[gpu] golden objective {obj!r} info {model.solve_info()}")
+    assert obj == pytest.approx(GOLDEN_OBJECTIVE, rel=1e-4)
+
+
+def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
+    _log("[gpu] golden field vs oracle")
+    diel, _ = golden_inputs
+    a = O.wrapper_args([22, 22, 10], 15.0, diel, [1.01 + 0j, 5.96282 + 3.80423e-7j], [0, 0, 1], [1, 0, 0], 542,
+                       symmetry_axes=[10.5, 10.5], tables=tables)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a, workers=4)
+    p0 = np.zeros(3 * 4840, complex)
+    g.solveElectricField(p0, 100000, 1e-5)
+    o.solveElectricField(p0, 100000, 1e-5)
+    info = g.solveInfo(0)
+    _log(f"[gpu] iterations gpu {info['iterations']} oracle {o.info.iterations}")
+    assert abs(info["iterations"] - o.info.iterations) <= 1
+    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
+    assert g.calculateObjective() == pytest.approx(o.calculateObjective(), rel=1e-8)
+    # tightly converged: both solve the same system
+    g.solveElectricField(p0, 100000, 1e-11)
+    o.solveElectricField(p0, 100000, 1e-11)
+    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-9
+
+
+def test_refined_solve_accuracy(dm, O, tables):
+    """complex64 transforms/Krylov with FP64 reliable updates: the true FP64
+    residual meets the tolerance and E is close to the FP64 solution."""
+    _log("[gpu] refined solve 16^3")
+    a = _args(O, tables, (16, 16, 16), method="LDR", seed=0)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    p0 = np.zeros(3 * o.N, complex)
+    o.solveElectricField(p0, 100000, 1e-12)
+    e_ref = o.getElectricField()
+    g.setSolverPrecision("fp32_refined")
+    g.solveElectricField(p0, 100000, 1e-5)
+    info = g.solveInfo(0)
+    e32 = g.getElectricField()
+    g.setSolverPrecision("fp64")
+    g.solveElectricField(p0, 100000, 1e-5)
+    e64 = g.getElectricField()
+    err32, err64 = _rel(e32, e_ref), _rel(e64, e_ref)
+    _log(f"[gpu] refined info {info}; E err refined {err32:.3e} fp64 {err64:.3e}")
+    assert info["status"] == "converged" and info["error"] <= 1e-5
+    # the true residual of the returned P, recomputed by the oracle in FP64
+    g.setSolverPrecision("fp32_refined")
+    g.solveElectricField(p0, 100000, 1e-5)
+    P = g.getPolarization()
+    r = o.E - o.apply(P)
+    assert np.linalg.norm(r) / np.linalg.norm(o.E) <= 1.0001e-5
+    assert err32 < 3 * max(err64, 1e-6)
+
+
+@pytest.mark.parametrize("mode", ["reference", "exact"])
+def test_gradients_match_oracle(dm, O, tables, mode):
+    _log(f"[gpu] gradients {mode}")
+    a = _args(O, tables, (8, 8, 3), method="FCD", maxmn=2, symmetry="4fold", seed=5)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    p0 = np.zeros(3 * o.N, complex)
+    g.setGradientMode(mode)
+    g.solveElectricField(p0, 100000, 1e-12)
+    o.solveElectricField(p0, 100000, 1e-12)
+    obj = g.calculateObjective()
+    assert obj == pytest.approx(o.calculateObjective(), rel=1e-10)
+    grad = g.calculateGradients(1e-3, obj, 100000, 1e-12)
+    if mode == "reference":
+        want = o.calculateGradients(1e-3, obj, 100000, 1e-12)
+        assert _rel(grad, want) < 1e-6
+    else:
+        want = o.exact_gradient(tol=1e-12)
+        assert _rel(grad, want) < 1e-8
+    if mode == "reference":
+        # side effect of the reference: P now holds the adjoint solution
+        assert _rel(g.getPolarization(), o.P) < 1e-6
+
+
+def test_parameters_round_trip(dm, O, tables):
+    _log("[gpu] parameters round trip")
+    a = _args(O, tables, (6, 6, 2), symmetry="4fold", seed=2)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    assert np.array_equal(g.getParameters(), o.getParameters())
+    assert np.array_equal(g.getDielectrics(), o.getDielectrics())
+    newp = np.linspace(0, 1, o.nfpara)
+    g.setParameters(newp)
+    o.setParameters(newp)
+    assert np.array_equal(g.getDielectrics(), o.getDielectrics())
+    assert np.allclose(g.getInverseAlpha(), o.al[0::3], rtol=1e-14)
+    with pytest.raises(RuntimeError):
+        g.setParameters(newp[:-1])
+
+
+def test_nonconvergence_reports_status(dm, O, tables):
+    _log("[gpu] maxit status")
+    a = _args(O, tables, (8, 8, 8), seed=1)
+    g = dm.DDAModel(*a)
+    g.solveElectricField(np.zeros(3 * 512, complex), 2, 1e-12)
+    info = g.solveInfo(0)
+    assert info["status"] == "maxit" and info["iterations"] == 2
+
+
+def test_against_committed_fixture(dm, O, tables):
+    """Regression against tests/golden/small_cases.npz (written by
+    tests/golden/make_golden.py from the oracle)."""
+    path = os.path.join(GOLDEN, "small_cases.npz")
+    data = np.load(path)
+    _log("[gpu] committed fixtures")
+    a = _args(O, tables, (6, 5, 4), method="FCD", maxmn=3, seed=21)
+    g = dm.DDAModel(*a)
+    assert _rel(g.applyOperator(data["x"]), data["y_fcd"]) < 1e-11
+    g.solveElectricField(np.zeros(360, complex), 100000, 1e-11)
+    assert _rel(g.getElectricField(), data["e_fcd"]) < 1e-8

@@ -1,0 +1,71 @@
+// Device-visible descriptors of one linear system ("job") and its Krylov state.
+#pragma once
+
+#include "common.h"
+#include "fft_core.h"
+
+namespace tdda {
+
+// Solver status codes (device and host)
+enum JobStatus : int {
+    JOB_ACTIVE = 0,
+    JOB_CONVERGED = 1,
+    JOB_BREAKDOWN = 2,   // <r0, r_prev> == 0 (DDAModel.cpp:952-970)
+    JOB_MAXIT = 3,       // ran out of iterations (DDAModel.cpp:1005-1008)
+    JOB_IDLE = 4,        // not started / retired
+};
+
+// BiCGSTAB scalars of one job, all FP64 (DDAModel.cpp:920-1004 names:
+// alpha, beta, zeta; rho = <r0, r>).
+struct KScalars {
+    double2 alpha, beta, zeta;
+    double2 rho;        // <r0, r_k>, numerator of alpha in the current step
+    double2 rho_prev;   // <r0, r_{k-1}>  (the reference's r0.dot(rl))
+    double enorm2;      // ||rhs||^2
+    double rnorm2;      // ||r||^2 of the latest residual (recursive or true)
+    double maxr2;       // reliable-update reference level
+    double tol;         // relative tolerance
+    double ru_delta;    // reliable-update trigger ratio (refined mode)
+    double err;         // final ||r||/||rhs||
+    int iter;           // loop index of the step in flight (-1: initial step)
+    int maxit;
+    int status;         // JobStatus
+    int need_ru;        // refined mode: true-residual recompute pending
+    int refined;        // 1: FP32 Krylov with FP64 reliable updates
+    int iterations;     // reported iteration count (reference ITERATION)
+    int n_ru;           // number of reliable updates done
+    int pad;
+};
+
+// One system (diag(al) + A) P = rhs on an Nx x Ny x Nz box.
+struct GridJob {
+    int nx, ny, nz, ndip;
+    int lx, ly, lz;
+    int hx, hy, hz;            // L/2 + 1: extents of the octant spectrum
+    FftPlan px, py, pz;
+    int bx32, by32, bz32;      // pencils (x: rows) per tile, c64 launches
+    int bx64, by64, bz64;      // same for c128 launches
+    int tx32, ty32, tz32;      // tiles per pass, c64
+    int tx64, ty64, tz64;      // tiles per pass, c128
+    int tupd;                  // tiles of the elementwise update kernel
+    const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
+    // operator data
+    const float2* green32;     // [6][hz][hy][hx]
+    const double2* green64;
+    const float2* al32;        // [ndip] 1/alpha
+    const double2* al64;
+    // work grid [3][nz][ly][lx]; the c64 view aliases the c128 buffer
+    void* grid;
+    // Krylov vectors, 3*ndip entries each, Krylov precision (float2 or double2)
+    void *r, *r0, *p, *v, *t, *s;
+    double2* P;                // solution, FP64
+    const double2* rhs;        // right-hand side, FP64
+    // plain operator application y = A x + al x (tests, gradients)
+    const void* xin;
+    void* yout;
+    KScalars* ks;
+    double* partial;           // [max tiles][6]
+    unsigned* counter;
+};
+
+}  // namespace tdda

@@ -1,0 +1,26 @@
+// Green's-tensor setup kernels (FP64).
+#pragma once
+
+#include "common.h"
+
+namespace tdda {
+
+struct GreenParams {
+    int nx, ny, nz;
+    double d;        // dipole spacing (nm)
+    double K;        // 2 pi nback / lambda
+    int method;      // 0 FCD, 1 LDR
+    int maxm, maxn;  // image range
+    double lm, ln;   // image displacement (nm)
+    long nsici;      // Si/Ci table length
+    double delta;    // Si/Ci table spacing assumed by the lookup
+};
+
+// octant tensor [6][nz][ny][nx]
+void launch_green_octant(const GreenParams& P, const double* si, const double* ci, double2* out, hipStream_t st);
+// one parity-reduced DFT axis: in [6][outer][no][inner] -> out [6][outer][h][inner]
+void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
+                      const double2* tw, hipStream_t st);
+void launch_to_c64(const double2* in, float2* out, long n, hipStream_t st);
+
+}  // namespace tdda

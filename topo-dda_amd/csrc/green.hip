@@ -1,0 +1,227 @@
+// Green's-tensor setup in FP64 (AProductCore::AProductCore, src/AProductCore.cpp:12-298).
+//
+// 1. k_green_*: the damped, lattice-summed interaction tensor A(o) for the
+//    octant offsets o = (ox,oy,oz) >= 0 (AProductCore.cpp:63-244, 335-458).
+//    Every component is even or odd in each axis, and the image sum is
+//    symmetric in m -> -m, n -> -n, so the octant determines the full grid.
+// 2. k_green_dft (x3): the spectrum on the padded grid restricted to
+//    0 <= k <= L/2 per axis: Ghat(k) = G(0) + sum_{o>0} G(o) 2cos(2 pi k o/L)
+//    for an even axis, sum_{o>0} G(o) (-2i) sin(2 pi k o/L) for an odd one.
+//    This is the DFT of the zero-padded, wrapped grid the reference builds
+//    (AProductCore.cpp:91-244 + 262-279) evaluated directly; k > L/2 follows
+//    from Ghat(L-k) = +-Ghat(k).
+#include "green.h"
+
+namespace tdda {
+
+// parity under x->-x, y->-y, z->-z of (00,01,02,11,12,22)
+__constant__ int c_odd[6][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 0, 0}, {0, 1, 1}, {0, 0, 0}};
+
+// SiCi.cpp:12-44 table lookup (abscissae (p+1)*delta; past the end -> last entry).
+__device__ inline double sici_lookup(const double* __restrict__ t, long n, double delta, double x) {
+    long p1 = (long)(floor(x / delta) - 1.0);
+    long p2 = (long)(ceil(x / delta) - 1.0);
+    if (p2 >= n) return t[n - 1];
+    if (p1 < 0) p1 = 0;   // r >= d keeps x >= pi - K d > delta; clamp only guards memory
+    if (p2 < 0) p2 = 0;
+    const double v1 = t[p1], v2 = t[p2];
+    const double a1 = (double)(p1 + 1) * delta, a2 = (double)(p2 + 1) * delta;
+    return v1 + ((v2 - v1) / (a2 - a1)) * (x - a1);
+}
+
+struct Sym6 {
+    double re[6], im[6];
+};
+
+// AProductCore.cpp:351-386 (point dipole)
+__device__ inline void ldr_tensor(double K, double x, double y, double z, Sym6& s) {
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double r = sqrt(xx + yy + zz);
+    for (int c = 0; c < 6; ++c) { s.re[c] = 0.0; s.im[c] = 0.0; }
+    if (r == 0.0) return;
+    const double r2 = r * r, r3 = r2 * r;
+    double sk, ck;
+    sincos(K * r, &sk, &ck);
+    const double pre_re = ck / r3, pre_im = sk / r3;
+    const double c2_re = 1.0 / r2, c2_im = -K * r / r2;
+    const double c3_re = K * K - 3.0 * c2_re, c3_im = -3.0 * c2_im;
+    const double k2 = K * K;
+    const double in_re[6] = {c2_re * (r2 - 3 * xx) - k2 * (yy + zz), x * y * c3_re, z * x * c3_re,
+                             c2_re * (r2 - 3 * yy) - k2 * (zz + xx), y * z * c3_re,
+                             c2_re * (r2 - 3 * zz) - k2 * (xx + yy)};
+    const double in_im[6] = {c2_im * (r2 - 3 * xx), x * y * c3_im, z * x * c3_im, c2_im * (r2 - 3 * yy),
+                             y * z * c3_im, c2_im * (r2 - 3 * zz)};
+    for (int c = 0; c < 6; ++c) {
+        s.re[c] = pre_re * in_re[c] - pre_im * in_im[c];
+        s.im[c] = pre_re * in_im[c] + pre_im * in_re[c];
+    }
+}
+
+// AProductCore.cpp:388-458 (filtered coupled dipole, Kf = pi/d)
+__device__ inline void fcd_tensor(double K, double d, const double* __restrict__ si, const double* __restrict__ ci,
+                                  long nt, double delta, double x, double y, double z, Sym6& s) {
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double r = sqrt(xx + yy + zz);
+    for (int c = 0; c < 6; ++c) { s.re[c] = 0.0; s.im[c] = 0.0; }
+    if (r == 0.0) return;
+    const double r2 = r * r, r3 = r2 * r;
+    const double kf = M_PI / d;
+    const double ap = (kf + K) * r, am = (kf - K) * r;
+    const double ci_p = sici_lookup(ci, nt, delta, ap);
+    const double ci_m = sici_lookup(ci, nt, delta, am);
+    const double si_p = sici_lookup(si, nt, delta, ap);
+    const double si_m = sici_lookup(si, nt, delta, am);
+    double sin_p, cos_p, sin_m, cos_m, skr, ckr, skf, ckf;
+    sincos(ap, &sin_p, &cos_p);
+    sincos(am, &sin_m, &cos_m);
+    const double kr = K * r;
+    sincos(kr, &skr, &ckr);
+    sincos(kf * r, &skf, &ckf);
+    const double c1_p = cos_p / r, c1_m = cos_m / r, s1_p = sin_p / r, s1_m = sin_m / r;
+    const double c2_p = (-ap * sin_p - cos_p) / r2, c2_m = (-am * sin_m - cos_m) / r2;
+    const double s2_p = (ap * cos_p - sin_p) / r2, s2_m = (am * cos_m - sin_m) / r2;
+    const double cc_re = ci_m - ci_p, cc_im = M_PI;
+    const double dc1 = c1_m - c1_p, dc2 = c2_m - c2_p;
+    const double ss = si_p + si_m, ss1 = s1_p + s1_m, ss2 = s2_p + s2_m;
+    const double gf_re = (skr * cc_re + ckr * ss) / (M_PI * r);
+    const double gf_im = (skr * cc_im) / (M_PI * r);
+    const double q1_re = (kr * ckr - skr) * cc_re - (kr * skr + ckr) * ss + r * skr * dc1 + r * ckr * ss1;
+    const double q1_im = (kr * ckr - skr) * cc_im;
+    const double q2_re = (-K * kr * skr) * cc_re - (K * kr * ckr) * ss + (2 * kr * ckr) * dc1 -
+                         (2 * kr * skr) * ss1 + r * skr * dc2 + r * ckr * ss2;
+    const double q2_im = (-K * kr * skr) * cc_im;
+    const double g1_re = q1_re / (M_PI * r2), g1_im = q1_im / (M_PI * r2);
+    const double g2_re = (r * q2_re - 2.0 * q1_re) / (M_PI * r3), g2_im = (r * q2_im - 2.0 * q1_im) / (M_PI * r3);
+    const double hr = (skf - kf * r * ckf) / (2.0 * M_PI * M_PI * r3);
+    const double t1_re = K * K * gf_re + g1_re / r + (4.0 * M_PI / 3.0) * hr;
+    const double t1_im = K * K * gf_im + g1_im / r;
+    const double t2_re = (g2_re - g1_re / r) / r2, t2_im = (g2_im - g1_im / r) / r2;
+    const double w[6] = {xx, x * y, z * x, yy, y * z, zz};
+    const double dg[6] = {1.0, 0.0, 0.0, 1.0, 0.0, 1.0};
+    for (int c = 0; c < 6; ++c) {
+        s.re[c] = -(dg[c] * t1_re + w[c] * t2_re);
+        s.im[c] = -(dg[c] * t1_im + w[c] * t2_im);
+    }
+}
+
+// A_dic_generator(x,y,z,m,n): shifted image times exp(-(0.01 K r)^4)
+__device__ inline void image_term(const GreenParams& P, const double* si, const double* ci, double x, double y,
+                                  double z, int m, int n, Sym6& t) {
+    const double xs = x + m * P.lm, ys = y + n * P.ln;
+    if (P.method == 0)
+        fcd_tensor(P.K, P.d, si, ci, P.nsici, P.delta, xs, ys, z, t);
+    else
+        ldr_tensor(P.K, xs, ys, z, t);
+    const double rr = sqrt(xs * xs + ys * ys + z * z);
+    const double q = 0.01 * P.K * rr;
+    const double q2 = q * q;
+    const double damp = exp(-(q2 * q2));
+    for (int c = 0; c < 6; ++c) { t.re[c] *= damp; t.im[c] *= damp; }
+}
+
+// One thread per octant offset (no images).
+__global__ void __launch_bounds__(256) k_green_single(GreenParams P, const double* __restrict__ si,
+                                                     const double* __restrict__ ci, double2* __restrict__ out) {
+    const long n = (long)P.nx * P.ny * P.nz;
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= n) return;
+    const int ox = cell % P.nx, oy = (cell / P.nx) % P.ny, oz = cell / ((long)P.nx * P.ny);
+    Sym6 t;
+    image_term(P, si, ci, P.d * ox, P.d * oy, P.d * oz, 0, 0, t);
+    for (int c = 0; c < 6; ++c) out[c * n + cell] = make_double2(t.re[c], t.im[c]);
+}
+
+// One workgroup per octant offset; threads split the (2MAXm+1)(2MAXn+1) images
+// (m outer, n inner order within a thread) and reduce in a fixed tree.
+__global__ void __launch_bounds__(256) k_green_lattice(GreenParams P, const double* __restrict__ si,
+                                                      const double* __restrict__ ci, double2* __restrict__ out) {
+    const long n = (long)P.nx * P.ny * P.nz;
+    const long cell = blockIdx.x;
+    const int ox = cell % P.nx, oy = (cell / P.nx) % P.ny, oz = cell / ((long)P.nx * P.ny);
+    const double x = P.d * ox, y = P.d * oy, z = P.d * oz;
+    const int nn = 2 * P.maxn + 1;
+    const int nimg = (2 * P.maxm + 1) * nn;
+    double acc[12];
+    for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+    for (int i = threadIdx.x; i < nimg; i += blockDim.x) {
+        const int m = i / nn - P.maxm, nimg_n = i % nn - P.maxn;
+        Sym6 t;
+        image_term(P, si, ci, x, y, z, m, nimg_n, t);
+        for (int c = 0; c < 6; ++c) { acc[2 * c] += t.re[c]; acc[2 * c + 1] += t.im[c]; }
+    }
+    __shared__ double red[12][256];
+    for (int c = 0; c < 12; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 12; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 6; ++c) {
+            // odd components vanish on their odd axes' zero plane by symmetry
+            const bool zero = (c_odd[c][0] && ox == 0) || (c_odd[c][1] && oy == 0) || (c_odd[c][2] && oz == 0);
+            out[c * n + cell] = zero ? make_double2(0.0, 0.0) : make_double2(red[2 * c][0], red[2 * c + 1][0]);
+        }
+    }
+}
+
+// out[c][a][k][i] = sum_o W_c(k,o) in[c][a][o][i], 0 <= k < h, 0 <= o < no
+__global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
+                                                   int outer, int no, int h, int inner, int L, int axis,
+                                                   const double2* __restrict__ tw) {
+    const long total = 6L * outer * h * inner;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int i = idx % inner;
+    const int k = (idx / inner) % h;
+    const long a = (idx / ((long)inner * h)) % outer;
+    const int c = idx / ((long)inner * h * outer);
+    const bool odd = c_odd[c][axis] != 0;
+    const double2* src = in + ((long)c * outer + a) * no * inner + i;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int o = 0; o < no; ++o) {
+        const double2 v = src[(long)o * inner];
+        const int m = (int)(((long)k * o) % L);
+        const double cs = tw[m].x, sn = -tw[m].y;   // tw[m] = exp(-2 pi i m / L)
+        if (!odd) {
+            const double w = o == 0 ? 1.0 : 2.0 * cs;
+            acc.x += w * v.x;
+            acc.y += w * v.y;
+        } else if (o > 0) {
+            const double s2 = 2.0 * sn;   // (-i s2) v
+            acc.x += s2 * v.y;
+            acc.y -= s2 * v.x;
+        }
+    }
+    out[(((long)c * outer + a) * h + k) * inner + i] = acc;
+}
+
+__global__ void k_to_c64(const double2* __restrict__ in, float2* __restrict__ out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+void launch_green_octant(const GreenParams& P, const double* si, const double* ci, double2* out, hipStream_t st) {
+    const long n = (long)P.nx * P.ny * P.nz;
+    if (P.maxm == 0 && P.maxn == 0)
+        hipLaunchKernelGGL(k_green_single, dim3((n + 255) / 256), dim3(256), 0, st, P, si, ci, out);
+    else
+        hipLaunchKernelGGL(k_green_lattice, dim3(n), dim3(256), 0, st, P, si, ci, out);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
+                      const double2* tw, hipStream_t st) {
+    const long total = 6L * outer * h * inner;
+    hipLaunchKernelGGL(k_green_dft, dim3((total + 255) / 256), dim3(256), 0, st, in, out, outer, no, h, inner, L,
+                       axis, tw);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_to_c64(const double2* in, float2* out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_c64, dim3((n + 255) / 256), dim3(256), 0, st, in, out, n);
+    TDDA_HIP(hipGetLastError());
+}
+
+}  // namespace tdda

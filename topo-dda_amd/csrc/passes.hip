@@ -1,0 +1,772 @@
+// FFT-convolution matvec passes and fused BiCGSTAB kernels (gfx950).
+//
+// The operator y = A u + al*u (AProductCore::Aproduct + DDAModel::Aproductwithalb,
+// src/AProductCore.cpp:461-515, src/DDAModel.cpp:1194-1203) is evaluated on an
+// Lx*Ly*Lz grid (L >= 2N-1 per axis, 2/3/5-smooth) in five passes that keep only
+// the non-zero part of the zero-padded grid resident:
+//
+//   X fwd : dipoles (x < Nx) -> rows (z < Nz, y < Ny, all x)        [fused vector op]
+//   Y fwd : rows y < Ny -> all y
+//   Z     : z < Nz -> FFT -> 3x3 symmetric Green's multiply -> IFFT -> z < Nz
+//   Y inv : all y -> y < Ny
+//   X inv : all x -> dipoles (x < Nx), /M, + al*u                    [fused epilogue + dots]
+//
+// The work grid is [3][Nz][Ly][Lx] (component, z, y, x; x fastest).  Every
+// launch covers a batch of jobs; block b handles (job, tile) = map[b].  Tiles
+// depend only on a job's own shape, and reductions are per job in a fixed
+// order, so a job's result does not depend on what else is in the batch.
+#include "passes.h"
+
+namespace tdda {
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+template <typename Real> struct Tile;
+template <> struct Tile<float> {
+    __device__ static int bx(const GridJob& J) { return J.bx32; }
+    __device__ static int by(const GridJob& J) { return J.by32; }
+    __device__ static int bz(const GridJob& J) { return J.bz32; }
+    __device__ static int tx(const GridJob& J) { return J.tx32; }
+    __device__ static const float2* green(const GridJob& J) { return J.green32; }
+    __device__ static const float2* al(const GridJob& J) { return J.al32; }
+};
+template <> struct Tile<double> {
+    __device__ static int bx(const GridJob& J) { return J.bx64; }
+    __device__ static int by(const GridJob& J) { return J.by64; }
+    __device__ static int bz(const GridJob& J) { return J.bz64; }
+    __device__ static int tx(const GridJob& J) { return J.tx64; }
+    __device__ static const double2* green(const GridJob& J) { return J.green64; }
+    __device__ static const double2* al(const GridJob& J) { return J.al64; }
+};
+
+__device__ inline bool gate_open(const GridJob& J, int gate) {
+    if (gate == GATE_ALWAYS) return true;
+    const KScalars* ks = J.ks;
+    if (ks->status != JOB_ACTIVE) return false;
+    if (gate == GATE_RU) return ks->need_ru != 0;
+    return true;
+}
+
+// Deterministic block reduction of NV doubles; thread 0 gets the sums.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV]) {
+    __shared__ double red[NV][FFT_THREADS];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = FFT_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = red[k][0];
+}
+
+// Publish this block's partial sums; returns true (in every thread) for the
+// block that completes the job.  That block then sums all partials in tile
+// order (fixed, hence deterministic) through sum_partials().
+template <int NV>
+__device__ inline bool publish_partials(const GridJob& J, int tile, int ntiles, const double (&v)[NV]) {
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) J.partial[(long)tile * 6 + k] = v[k];
+        __threadfence();
+        const unsigned prev = atomicAdd(J.counter, 1u);
+        s_last = (prev + 1 == (unsigned)ntiles) ? 1 : 0;
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+
+template <int NV>
+__device__ inline void sum_partials(const GridJob& J, int ntiles, double (&out)[NV]) {
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = 0.0;
+    for (int t = 0; t < ntiles; ++t) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            out[k] += __hip_atomic_load(&J.partial[(long)t * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ----------------------------------------------------------------------------
+// scalar logic (runs in one thread of the last block), DDAModel.cpp:945-1004
+// ----------------------------------------------------------------------------
+__device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2) {
+    ks->rnorm2 = rn2;
+    const double rel = sqrt(rn2 / ks->enorm2);
+    const int it = ks->iter;
+    if (it >= 0 && rel <= ks->tol) {      // ||r||/||E|| <= tol after a loop step
+        ks->status = JOB_CONVERGED;
+        ks->iterations = it + 1;
+        ks->err = rel;
+        return;
+    }
+    const int next = it + 1;
+    if (next >= ks->maxit) {
+        ks->status = JOB_MAXIT;
+        ks->iterations = ks->maxit;
+        ks->err = rel;
+        return;
+    }
+    if (ks->rho.x == 0.0 && ks->rho.y == 0.0) {   // r0.dot(rl) == 0 at the top of the loop
+        ks->status = JOB_BREAKDOWN;
+        ks->iterations = next + 1;
+        ks->err = rel;
+        return;
+    }
+    // beta = (alpha/zeta) * <r0,r> / <r0,rl>
+    ks->beta = ddiv(dcmul(ddiv(ks->alpha, ks->zeta), rho_new), ks->rho);
+    ks->rho_prev = ks->rho;
+    ks->rho = rho_new;
+    ks->iter = next;
+}
+
+// ----------------------------------------------------------------------------
+// X forward pass
+// ----------------------------------------------------------------------------
+template <typename Real, typename KReal, int MODE>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
+    using C = cx_t<Real>;
+    using KC = cx_t<KReal>;
+    constexpr int CAP = Caps<Real>::XY;
+    extern __shared__ __align__(16) unsigned char smem_x[];
+    C* lds = reinterpret_cast<C*>(smem_x);
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    if (!gate_open(J, gate)) return;
+    const int bx = Tile<Real>::bx(J);
+    const int nx = J.nx, ny = J.ny, lx = J.lx;
+    const int nrows = ny * J.nz;
+    const int q0 = jt.y * bx;
+    const int nb = min(bx, nrows - q0);
+    const int np = 3 * nb;
+    const int bp = np | 1;
+    const int nel = 3 * nx * nb;
+    double2 alpha = make_double2(0, 0), beta = make_double2(0, 0), zeta = make_double2(0, 0);
+    if (MODE == XF_P) { beta = J.ks->beta; zeta = J.ks->zeta; }
+    if (MODE == XF_T) { alpha = J.ks->alpha; }
+    for (int e = threadIdx.x; e < nel; e += FFT_THREADS) {
+        const int c = e % 3;
+        const int rem = e / 3;
+        const int x = rem % nx;
+        const int b = rem / nx;
+        long vi;
+        if (J.box_map == nullptr) {
+            vi = 3L * nx * q0 + e;
+        } else {
+            const int dip = J.box_map[(long)(q0 + b) * nx + x];
+            vi = dip >= 0 ? 3L * dip + c : -1;
+        }
+        C u = mk<C>(0.0, 0.0);
+        if (vi >= 0) {
+            if (MODE == XF_PLAIN) {
+                u = static_cast<const C*>(J.xin)[vi];
+            } else if (MODE == XF_SOL) {
+                u = from_d<C>(J.P[vi]);
+            } else if (MODE == XF_P) {
+                KC* r = static_cast<KC*>(J.r);
+                KC* p = static_cast<KC*>(J.p);
+                KC* v = static_cast<KC*>(J.v);
+                const double2 w = csub(to_d(p[vi]), dcmul(zeta, to_d(v[vi])));   // p - u, u = zeta v
+                const KC pk = from_d<KC>(cadd(to_d(r[vi]), dcmul(beta, w)));
+                p[vi] = pk;
+                u = from_d<C>(to_d(pk));
+            } else if (MODE == XF_T) {
+                KC* r = static_cast<KC*>(J.r);
+                KC* v = static_cast<KC*>(J.v);
+                KC* t = static_cast<KC*>(J.t);
+                const KC tk = from_d<KC>(csub(to_d(r[vi]), dcmul(alpha, to_d(v[vi]))));
+                t[vi] = tk;
+                u = from_d<C>(to_d(tk));
+            }
+        }
+        lds[x * bp + 3 * b + c] = u;
+    }
+    __syncthreads();
+    lds_fft<-1, CAP, 1>(J.px, lds, bp, np, nx, tw);
+    C* g = static_cast<C*>(J.grid);
+    const long S = (long)J.nz * J.ly * lx;
+    const int nout = np * lx;
+    for (int e = threadIdx.x; e < nout; e += FFT_THREADS) {
+        const int x = e % lx;
+        const int rem = e / lx;
+        const int b = rem % nb;
+        const int c = rem / nb;
+        const int q = q0 + b;
+        const int z = q / ny;
+        const int y = q - z * ny;
+        g[c * S + ((long)z * J.ly + y) * lx + x] = lds[x * bp + 3 * b + c];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// X inverse pass with the operator epilogue
+// ----------------------------------------------------------------------------
+template <typename Real, typename KReal, int MODE>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
+    using C = cx_t<Real>;
+    using KC = cx_t<KReal>;
+    constexpr int CAP = Caps<Real>::XY;
+    extern __shared__ __align__(16) unsigned char smem_xi[];
+    C* lds = reinterpret_cast<C*>(smem_xi);
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    if (!gate_open(J, gate)) return;
+    const int bx = Tile<Real>::bx(J);
+    const int nx = J.nx, ny = J.ny, lx = J.lx;
+    const int nrows = ny * J.nz;
+    const int q0 = jt.y * bx;
+    const int nb = min(bx, nrows - q0);
+    const int np = 3 * nb;
+    const int bp = np | 1;
+    const C* g = static_cast<const C*>(J.grid);
+    const long S = (long)J.nz * J.ly * lx;
+    const int nin = np * lx;
+    for (int e = threadIdx.x; e < nin; e += FFT_THREADS) {
+        const int x = e % lx;
+        const int rem = e / lx;
+        const int b = rem % nb;
+        const int c = rem / nb;
+        const int q = q0 + b;
+        const int z = q / ny;
+        const int y = q - z * ny;
+        lds[x * bp + 3 * b + c] = g[c * S + ((long)z * J.ly + y) * lx + x];
+    }
+    __syncthreads();
+    lds_fft<1, CAP, 1>(J.px, lds, bp, np, lx, tw);
+
+    const Real inv_m = (Real)(1.0 / ((double)J.lx * (double)J.ly * (double)J.lz));
+    const C* al = Tile<Real>::al(J);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int nel = 3 * nx * nb;
+    for (int e = threadIdx.x; e < nel; e += FFT_THREADS) {
+        const int c = e % 3;
+        const int rem = e / 3;
+        const int x = rem % nx;
+        const int b = rem / nx;
+        long vi;
+        if (J.box_map == nullptr) {
+            vi = 3L * nx * q0 + e;
+        } else {
+            const int dip = J.box_map[(long)(q0 + b) * nx + x];
+            vi = dip >= 0 ? 3L * dip + c : -1;
+        }
+        if (vi < 0) continue;
+        const C conv = cscale(lds[x * bp + 3 * b + c], inv_m);
+        const C a = al[vi / 3];
+        if (MODE == XI_PLAIN) {
+            const C u = static_cast<const C*>(J.xin)[vi];
+            static_cast<C*>(J.yout)[vi] = cadd(conv, cmul(a, u));
+        } else if (MODE == XI_V) {
+            const KC* p = static_cast<const KC*>(J.p);
+            const KC* r0 = static_cast<const KC*>(J.r0);
+            const C y = cadd(conv, cmul(a, from_d<C>(to_d(p[vi]))));
+            const KC vk = from_d<KC>(to_d(y));
+            static_cast<KC*>(J.v)[vi] = vk;
+            const double2 d = cmulc(to_d(vk), to_d(r0[vi]));   // conj(r0) v
+            acc[0] += d.x;
+            acc[1] += d.y;
+        } else if (MODE == XI_S) {
+            const KC* t = static_cast<const KC*>(J.t);
+            const KC tk = t[vi];
+            const C y = cadd(conv, cmul(a, from_d<C>(to_d(tk))));
+            const KC sk = from_d<KC>(to_d(y));
+            static_cast<KC*>(J.s)[vi] = sk;
+            const double2 sd = to_d(sk), td = to_d(tk);
+            const double2 d = cmulc(td, sd);   // conj(s) t
+            acc[0] += d.x;
+            acc[1] += d.y;
+            acc[2] += sd.x * sd.x + sd.y * sd.y;
+        } else if (MODE == XI_RESID) {
+            const double2 y = cadd(to_d(conv), dcmul(to_d(a), J.P[vi]));
+            const double2 rr = csub(J.rhs[vi], y);
+            const KC rk = from_d<KC>(rr);
+            static_cast<KC*>(J.r)[vi] = rk;
+            const double2 d = cmulc(to_d(rk), to_d(static_cast<const KC*>(J.r0)[vi]));
+            acc[0] += d.x;
+            acc[1] += d.y;
+            acc[2] += rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    if (MODE == XI_PLAIN) return;
+    block_sum<3>(acc);
+    const int ntiles = Tile<Real>::tx(J);
+    if (!publish_partials<3>(J, jt.y, ntiles, acc)) return;
+    if (threadIdx.x != 0) return;
+    double sum[3];
+    sum_partials<3>(J, ntiles, sum);
+    KScalars* ks = J.ks;
+    const double2 dot = make_double2(sum[0], sum[1]);
+    if (MODE == XI_V) {
+        ks->alpha = ddiv(ks->rho, dot);                          // alpha = <r0,r>/<r0,Ap>
+    } else if (MODE == XI_S) {
+        ks->zeta = make_double2(dot.x / sum[2], dot.y / sum[2]);  // zeta = <At,t>/<At,At>
+    } else if (MODE == XI_RESID) {
+        ks->need_ru = 0;
+        ks->n_ru += 1;
+        ks->maxr2 = sum[2];
+        if (ks->status == JOB_ACTIVE) bicg_decide(ks, dot, sum[2]);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Y pass (strided pencils, global -> global through LDS)
+// ----------------------------------------------------------------------------
+template <typename Real, int DIR>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate) {
+    using C = cx_t<Real>;
+    constexpr int CAP = Caps<Real>::XY;
+    extern __shared__ __align__(16) unsigned char smem_y[];
+    C* lds = reinterpret_cast<C*>(smem_y);
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    if (!gate_open(J, gate)) return;
+    const int by = Tile<Real>::by(J);
+    const int lx = J.lx, ly = J.ly;
+    const int npen = 3 * J.nz * lx;
+    const int pl0 = jt.y * by;
+    const int np = min(by, npen - pl0);
+    const int bp = np | 1;
+    const int n_in = DIR < 0 ? J.ny : ly;
+    const int n_out = DIR < 0 ? ly : J.ny;
+    C* g = static_cast<C*>(J.grid);
+    const FftPlan& pl = J.py;
+    const int L = pl.L;
+    const cx_t<Real>* tw = twpool + pl.tw_off;
+    auto gaddr = [&](int b, int pos) -> long {
+        const int p = pl0 + b;
+        const int cz = p / lx;
+        const int x = p - cz * lx;
+        return ((long)cz * ly + pos) * lx + x;
+    };
+    auto gload = [&](int, int b, int pos) -> C { return pos < n_in ? g[gaddr(b, pos)] : mk<C>(0.0, 0.0); };
+    auto gstore = [&](int, int b, int pos, C v) { if (pos < n_out) g[gaddr(b, pos)] = v; };
+    auto lload = [&](int, int b, int pos) -> C { return lds[pos * bp + b]; };
+    auto lstore = [&](int, int b, int pos, C v) { lds[pos * bp + b] = v; };
+    const int nst = pl.nst;
+    if (nst == 1) {
+        io_stage_rt<DIR, CAP, 1, C>(pl.radix[0], np, L, 1, tw, gload, gstore, false);
+        return;
+    }
+    int ns = 1;
+    int R = stage_radix<DIR>(pl, 0);
+    io_stage_rt<DIR, CAP, 1, C>(R, np, L, ns, tw, gload, lstore, false);
+    __syncthreads();
+    ns *= R;
+    for (int s = 1; s < nst - 1; ++s) {
+        R = stage_radix<DIR>(pl, s);
+        lds_stage_rt<DIR, CAP, 1>(R, lds, bp, np, L, ns, tw);
+        ns *= R;
+    }
+    R = stage_radix<DIR>(pl, nst - 1);
+    io_stage_rt<DIR, CAP, 1, C>(R, np, L, ns, tw, lload, gstore, false);
+}
+
+// ----------------------------------------------------------------------------
+// Z pass: forward FFT, Green's-tensor multiply, inverse FFT, all in one kernel
+// ----------------------------------------------------------------------------
+// Green's spectrum symmetry: component (00,01,02,11,12,22) is odd in the axes
+// {}, {x,y}, {x,z}, {}, {y,z}, {} respectively, so only k <= L/2 per axis is
+// stored and Ghat(L-k) = +-Ghat(k).
+template <typename C>
+__device__ inline void green_apply(const C* __restrict__ G, const GridJob& J, int kx, int ky, int kz,
+                                   C& v0, C& v1, C& v2) {
+    int sx = 1, sy = 1, sz = 1;
+    if (2 * kx > J.lx) { kx = J.lx - kx; sx = -1; }
+    if (2 * ky > J.ly) { ky = J.ly - ky; sy = -1; }
+    if (2 * kz > J.lz) { kz = J.lz - kz; sz = -1; }
+    const long cs = (long)J.hz * J.hy * J.hx;
+    const long off = ((long)kz * J.hy + ky) * J.hx + kx;
+    const C g0 = G[off];
+    C g1 = G[cs + off];
+    C g2 = G[2 * cs + off];
+    const C g3 = G[3 * cs + off];
+    C g4 = G[4 * cs + off];
+    const C g5 = G[5 * cs + off];
+    if (sx * sy < 0) g1 = cscale(g1, -1);
+    if (sx * sz < 0) g2 = cscale(g2, -1);
+    if (sy * sz < 0) g4 = cscale(g4, -1);
+    const C a0 = v0, a1 = v1, a2 = v2;
+    v0 = cadd(cadd(cmul(g0, a0), cmul(g1, a1)), cmul(g2, a2));
+    v1 = cadd(cadd(cmul(g1, a0), cmul(g3, a1)), cmul(g4, a2));
+    v2 = cadd(cadd(cmul(g2, a0), cmul(g4, a1)), cmul(g5, a2));
+}
+
+// Last forward stage (radix R), multiply in registers, first inverse stage
+// (same radix, so every thread already holds its inputs).
+template <int R, int CAP, typename C, typename TW, typename LoadF, typename StoreF, typename MulF>
+__device__ inline void z_bridge(int np, int L, const TW* __restrict__ tw, LoadF load, StoreF store, MulF mul,
+                                bool from_lds) {
+    constexpr int U = Umax<R, CAP>::value;
+    StageRegs<R, U, 3, C> s;
+    const int ns = L / R;
+    stage_compute<R, -1, U, 3>(s, np, L, ns, tw, load);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (s.pos[u] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) mul(s.b[u], s.pos[u] + r * ns, s.v[0][u][r], s.v[1][u][r], s.v[2][u][r]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Dft<R, 1, C>::run(s.v[c][u]);
+            s.pos[u] *= R;   // inverse stage 0: outputs at j*R + r
+        }
+    }
+    if (from_lds) __syncthreads();
+    stage_store<R, U, 3>(s, 1, store);
+}
+
+template <int CAP, typename C, typename TW, typename LoadF, typename StoreF, typename MulF>
+__device__ inline void z_bridge_rt(int R, int np, int L, const TW* __restrict__ tw, LoadF load, StoreF store,
+                                   MulF mul, bool from_lds) {
+    switch (R) {
+        case 2: z_bridge<2, CAP, C>(np, L, tw, load, store, mul, from_lds); break;
+        case 3: z_bridge<3, CAP, C>(np, L, tw, load, store, mul, from_lds); break;
+        case 4: z_bridge<4, CAP, C>(np, L, tw, load, store, mul, from_lds); break;
+        case 5: z_bridge<5, CAP, C>(np, L, tw, load, store, mul, from_lds); break;
+        case 8: z_bridge<8, CAP, C>(np, L, tw, load, store, mul, from_lds); break;
+        default: break;
+    }
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate) {
+    using C = cx_t<Real>;
+    constexpr int CAP = Caps<Real>::Z;
+    extern __shared__ __align__(16) unsigned char smem_z[];
+    C* lds = reinterpret_cast<C*>(smem_z);
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    if (!gate_open(J, gate)) return;
+    const int bz = Tile<Real>::bz(J);
+    const int lx = J.lx, nz = J.nz;
+    const long plane = (long)J.ly * lx;
+    const int npen = J.ly * lx;
+    const int pl0 = jt.y * bz;
+    const int np = min(bz, npen - pl0);
+    const int bp = np | 1;
+    const long S = (long)nz * plane;
+    C* g = static_cast<C*>(J.grid);
+    const C* G = Tile<Real>::green(J);
+    const FftPlan& pl = J.pz;
+    const int L = pl.L;
+    const cx_t<Real>* tw = twpool + pl.tw_off;
+    auto gload = [&](int c, int b, int pos) -> C {
+        return pos < nz ? g[c * S + pos * plane + pl0 + b] : mk<C>(0.0, 0.0);
+    };
+    auto gstore = [&](int c, int b, int pos, C v) { if (pos < nz) g[c * S + pos * plane + pl0 + b] = v; };
+    auto lload = [&](int c, int b, int pos) -> C { return lds[(c * L + pos) * bp + b]; };
+    auto lstore = [&](int c, int b, int pos, C v) { lds[(c * L + pos) * bp + b] = v; };
+    auto mul = [&](int b, int kz, C& v0, C& v1, C& v2) {
+        const int p = pl0 + b;
+        const int ky = p / lx;
+        const int kx = p - ky * lx;
+        green_apply(G, J, kx, ky, kz, v0, v1, v2);
+    };
+    const int nst = pl.nst;
+    if (nst == 1) {
+        z_bridge_rt<CAP, C>(pl.radix[0], np, L, tw, gload, gstore, mul, false);
+        return;
+    }
+    // forward stage 0: global -> LDS
+    int R = pl.radix[0];
+    io_stage_rt<-1, CAP, 3, C>(R, np, L, 1, tw, gload, lstore, false);
+    __syncthreads();
+    int ns = R;
+    for (int s = 1; s < nst - 1; ++s) {
+        R = pl.radix[s];
+        lds_stage_rt<-1, CAP, 3>(R, lds, bp, np, L, ns, tw);
+        ns *= R;
+    }
+    // last forward stage + multiply + first inverse stage: LDS -> LDS
+    R = pl.radix[nst - 1];
+    z_bridge_rt<CAP, C>(R, np, L, tw, lload, lstore, mul, true);
+    __syncthreads();
+    ns = R;
+    for (int s = nst - 2; s >= 1; --s) {
+        R = pl.radix[s];
+        lds_stage_rt<1, CAP, 3>(R, lds, bp, np, L, ns, tw);
+        ns *= R;
+    }
+    // last inverse stage: LDS -> global (z < Nz only)
+    R = pl.radix[0];
+    io_stage_rt<1, CAP, 3, C>(R, np, L, ns, tw, lload, gstore, false);
+}
+
+// ----------------------------------------------------------------------------
+// BiCGSTAB update (DDAModel.cpp:981-986):  z = zeta r - alpha u, u = zeta v,
+// P += alpha p + z;  r <- t - zeta s;  partial <r0, r>, ||r||^2
+// ----------------------------------------------------------------------------
+template <typename KReal>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_update(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int gate) {
+    using KC = cx_t<KReal>;
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    if (!gate_open(J, gate)) return;
+    KScalars* ks = J.ks;
+    const double2 alpha = ks->alpha, zeta = ks->zeta;
+    const long n3 = 3L * J.ndip;
+    const long base = (long)jt.y * FFT_THREADS * UPD_EPT;
+    const KC* p = static_cast<const KC*>(J.p);
+    KC* r = static_cast<KC*>(J.r);
+    const KC* v = static_cast<const KC*>(J.v);
+    const KC* t = static_cast<const KC*>(J.t);
+    const KC* s = static_cast<const KC*>(J.s);
+    const KC* r0 = static_cast<const KC*>(J.r0);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < UPD_EPT; ++k) {
+        const long i = base + (long)k * FFT_THREADS + threadIdx.x;
+        if (i < n3) {
+            const double2 rv = to_d(r[i]);
+            const double2 u = dcmul(zeta, to_d(v[i]));
+            const double2 z = csub(dcmul(zeta, rv), dcmul(alpha, u));
+            J.P[i] = cadd(cadd(J.P[i], dcmul(alpha, to_d(p[i]))), z);
+            const KC rk = from_d<KC>(csub(to_d(t[i]), dcmul(zeta, to_d(s[i]))));
+            r[i] = rk;
+            const double2 rd = to_d(rk);
+            const double2 d = cmulc(rd, to_d(r0[i]));
+            acc[0] += d.x;
+            acc[1] += d.y;
+            acc[2] += rd.x * rd.x + rd.y * rd.y;
+        }
+    }
+    block_sum<3>(acc);
+    if (!publish_partials<3>(J, jt.y, J.tupd, acc)) return;
+    if (threadIdx.x != 0) return;
+    double sum[3];
+    sum_partials<3>(J, J.tupd, sum);
+    const double2 rho_new = make_double2(sum[0], sum[1]);
+    const double rn2 = sum[2];
+    if (ks->refined) {
+        const double rel2 = rn2 / ks->enorm2;
+        const bool trigger = rn2 <= ks->ru_delta * ks->ru_delta * ks->maxr2 ||
+                             rel2 <= ks->tol * ks->tol || ks->iter + 1 >= ks->maxit;
+        if (trigger && ks->iter >= 0) {
+            ks->need_ru = 1;     // true residual decides (k_xinv XI_RESID)
+            ks->rnorm2 = rn2;
+            return;
+        }
+        if (rn2 > ks->maxr2) ks->maxr2 = rn2;
+    }
+    bicg_decide(ks, rho_new, rn2);
+}
+
+// ----------------------------------------------------------------------------
+// initial state: r = rhs (- A P0 if warm), r0 = r, p = v = 0
+// ----------------------------------------------------------------------------
+template <typename KReal>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm) {
+    using KC = cx_t<KReal>;
+    const int2 jt = bmap[blockIdx.x];
+    const GridJob& J = jobs[jt.x];
+    const long n3 = 3L * J.ndip;
+    const long base = (long)jt.y * FFT_THREADS * UPD_EPT;
+    double acc[2] = {0.0, 0.0};
+    const KC zero = mk<KC>(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < UPD_EPT; ++k) {
+        const long i = base + (long)k * FFT_THREADS + threadIdx.x;
+        if (i < n3) {
+            const double2 b = J.rhs[i];
+            double2 rr = b;
+            if (warm) rr = csub(b, static_cast<const double2*>(J.yout)[i]);
+            const KC rk = from_d<KC>(rr);
+            static_cast<KC*>(J.r)[i] = rk;
+            static_cast<KC*>(J.r0)[i] = rk;
+            static_cast<KC*>(J.p)[i] = zero;
+            static_cast<KC*>(J.v)[i] = zero;
+            const double2 rd = to_d(rk);
+            acc[0] += rd.x * rd.x + rd.y * rd.y;
+            acc[1] += b.x * b.x + b.y * b.y;
+        }
+    }
+    block_sum<2>(acc);
+    if (!publish_partials<2>(J, jt.y, J.tupd, acc)) return;
+    if (threadIdx.x != 0) return;
+    double sum[2];
+    sum_partials<2>(J, J.tupd, sum);
+    KScalars* ks = J.ks;
+    ks->alpha = make_double2(0, 0);
+    ks->beta = make_double2(0, 0);
+    ks->zeta = make_double2(0, 0);
+    ks->rho = make_double2(sum[0], 0.0);
+    ks->rho_prev = make_double2(0, 0);
+    ks->enorm2 = sum[1];
+    ks->rnorm2 = sum[0];
+    ks->maxr2 = sum[0];
+    ks->iter = -1;
+    ks->need_ru = 0;
+    ks->n_ru = 0;
+    ks->iterations = 0;
+    ks->err = 0.0;
+    // a zero right-hand side has the exact solution P = 0
+    ks->status = (sum[1] == 0.0 && !warm) ? JOB_CONVERGED : JOB_ACTIVE;
+}
+
+// ----------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------
+template <typename Real, typename KReal>
+static void matvec_impl(const LaunchSet& ls, const cx_t<Real>* tw, int xf, int xi, int gate, hipStream_t st) {
+    const bool c64 = sizeof(Real) == 4;
+    const int2* mx = c64 ? ls.map_x32 : ls.map_x64;
+    const int2* my = c64 ? ls.map_y32 : ls.map_y64;
+    const int2* mz = c64 ? ls.map_z32 : ls.map_z64;
+    const int nx = c64 ? ls.n_x32 : ls.n_x64;
+    const int ny = c64 ? ls.n_y32 : ls.n_y64;
+    const int nz = c64 ? ls.n_z32 : ls.n_z64;
+    const size_t lx = c64 ? ls.lds_x32 : ls.lds_x64;
+    const size_t ly = c64 ? ls.lds_y32 : ls.lds_y64;
+    const size_t lz = c64 ? ls.lds_z32 : ls.lds_z64;
+    if (nx == 0) return;
+    const dim3 blk(FFT_THREADS);
+    switch (xf) {
+        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        default: throw Error(ERR_INVALID, "bad X-forward mode");
+    }
+    hipLaunchKernelGGL((k_ypass<Real, -1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, gate);
+    hipLaunchKernelGGL((k_zpass<Real>), dim3(nz), blk, lz, st, ls.jobs, mz, tw, gate);
+    hipLaunchKernelGGL((k_ypass<Real, 1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, gate);
+    switch (xi) {
+        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        default: throw Error(ERR_INVALID, "bad X-inverse mode");
+    }
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_matvec(const LaunchSet& ls, const TwiddlePools& tw, bool c64, int xf_mode, int xi_mode, int gate,
+                   hipStream_t st) {
+    if (c64) {
+        matvec_impl<float, float>(ls, tw.tw32, xf_mode, xi_mode, gate, st);
+    } else if (ls.refined) {
+        // FP64 transform feeding c64 Krylov vectors: reliable updates only
+        if (!(xf_mode == XF_SOL && xi_mode == XI_RESID) && !(xf_mode == XF_PLAIN && xi_mode == XI_PLAIN))
+            throw Error(ERR_INVALID, "FP64 transform in refined mode supports residual/plain only");
+        matvec_impl<double, float>(ls, tw.tw64, xf_mode, xi_mode, gate, st);
+    } else {
+        matvec_impl<double, double>(ls, tw.tw64, xf_mode, xi_mode, gate, st);
+    }
+}
+
+template <typename Real, typename KReal>
+static void matvec_timed_impl(const LaunchSet& ls, const cx_t<Real>* tw, hipStream_t st, int reps, double* ms5) {
+    const bool c64 = sizeof(Real) == 4;
+    const int2* mx = c64 ? ls.map_x32 : ls.map_x64;
+    const int2* my = c64 ? ls.map_y32 : ls.map_y64;
+    const int2* mz = c64 ? ls.map_z32 : ls.map_z64;
+    const int nx = c64 ? ls.n_x32 : ls.n_x64;
+    const int ny = c64 ? ls.n_y32 : ls.n_y64;
+    const int nz = c64 ? ls.n_z32 : ls.n_z64;
+    const size_t lx = c64 ? ls.lds_x32 : ls.lds_x64;
+    const size_t ly = c64 ? ls.lds_y32 : ls.lds_y64;
+    const size_t lz = c64 ? ls.lds_z32 : ls.lds_z64;
+    hipEvent_t ev[6];
+    for (auto& e : ev) TDDA_HIP(hipEventCreate(&e));
+    const dim3 blk(FFT_THREADS);
+    const int g = GATE_ALWAYS;
+    for (int r = 0; r < reps; ++r) {
+        TDDA_HIP(hipEventRecord(ev[0], st));
+        hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, g);
+        TDDA_HIP(hipEventRecord(ev[1], st));
+        hipLaunchKernelGGL((k_ypass<Real, -1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, g);
+        TDDA_HIP(hipEventRecord(ev[2], st));
+        hipLaunchKernelGGL((k_zpass<Real>), dim3(nz), blk, lz, st, ls.jobs, mz, tw, g);
+        TDDA_HIP(hipEventRecord(ev[3], st));
+        hipLaunchKernelGGL((k_ypass<Real, 1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, g);
+        TDDA_HIP(hipEventRecord(ev[4], st));
+        hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, g);
+        TDDA_HIP(hipEventRecord(ev[5], st));
+        TDDA_HIP(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.f;
+            TDDA_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            ms5[k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, hipStream_t st, int reps,
+                         double* ms5) {
+    if (c64)
+        matvec_timed_impl<float, float>(ls, tw.tw32, st, reps, ms5);
+    else if (ls.refined)
+        matvec_timed_impl<double, float>(ls, tw.tw64, st, reps, ms5);
+    else
+        matvec_timed_impl<double, double>(ls, tw.tw64, st, reps, ms5);
+}
+
+void launch_update(const LaunchSet& ls, int gate, hipStream_t st) {
+    if (ls.n_upd == 0) return;
+    if (ls.refined)
+        hipLaunchKernelGGL((k_update<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, gate);
+    else
+        hipLaunchKernelGGL((k_update<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, gate);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_init(const LaunchSet& ls, bool warm, hipStream_t st) {
+    if (ls.n_upd == 0) return;
+    if (ls.refined)
+        hipLaunchKernelGGL((k_init<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, (int)warm);
+    else
+        hipLaunchKernelGGL((k_init<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, (int)warm);
+    TDDA_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------
+// tile geometry
+// ----------------------------------------------------------------------------
+void plan_tiles(GridJob& J) {
+    auto cdiv = [](long a, long b) { return (int)((a + b - 1) / b); };
+    J.bx32 = std::max(1, Caps<float>::XY / (3 * J.lx));
+    J.by32 = std::max(1, Caps<float>::XY / J.ly);
+    J.bz32 = std::max(1, Caps<float>::Z / J.lz);
+    J.bx64 = std::max(1, Caps<double>::XY / (3 * J.lx));
+    J.by64 = std::max(1, Caps<double>::XY / J.ly);
+    J.bz64 = std::max(1, Caps<double>::Z / J.lz);
+    const long nrows = (long)J.ny * J.nz;
+    J.tx32 = cdiv(nrows, J.bx32);
+    J.ty32 = cdiv(3L * J.nz * J.lx, J.by32);
+    J.tz32 = cdiv((long)J.ly * J.lx, J.bz32);
+    J.tx64 = cdiv(nrows, J.bx64);
+    J.ty64 = cdiv(3L * J.nz * J.lx, J.by64);
+    J.tz64 = cdiv((long)J.ly * J.lx, J.bz64);
+    J.tupd = cdiv(3L * J.ndip, (long)FFT_THREADS * UPD_EPT);
+}
+
+size_t lds_bytes_x(const GridJob& J, bool c64) {
+    const int b = c64 ? J.bx32 : J.bx64;
+    return (size_t)J.lx * ((3 * b) | 1) * (c64 ? sizeof(float2) : sizeof(double2));
+}
+size_t lds_bytes_y(const GridJob& J, bool c64) {
+    const int b = c64 ? J.by32 : J.by64;
+    return (size_t)J.ly * (b | 1) * (c64 ? sizeof(float2) : sizeof(double2));
+}
+size_t lds_bytes_z(const GridJob& J, bool c64) {
+    const int b = c64 ? J.bz32 : J.bz64;
+    return (size_t)3 * J.lz * (b | 1) * (c64 ? sizeof(float2) : sizeof(double2));
+}
+
+}  // namespace tdda

@@ -1,0 +1,532 @@
+// Host runtime implementation: memory, plans, operators, systems, batches.
+#include "runtime.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace tdda {
+
+// ---------------------------------------------------------------------------
+// memory
+// ---------------------------------------------------------------------------
+static std::atomic<size_t> g_bytes{0};
+size_t device_bytes_in_use() { return g_bytes.load(); }
+
+void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    g_bytes += bytes;
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    g_bytes -= bytes;
+}
+
+hipStream_t default_stream() { return nullptr; }
+
+// ---------------------------------------------------------------------------
+// FFT lengths and plans
+// ---------------------------------------------------------------------------
+static bool smooth235(int L) {
+    for (int p : {2, 3, 5})
+        while (L % p == 0) L /= p;
+    return L == 1;
+}
+
+int choose_fft_length(int n) {
+    int L = std::max(2, 2 * n - 1);
+    while (!smooth235(L)) ++L;
+    return L;
+}
+
+std::vector<int> radix_sequence(int L) {
+    std::vector<int> r;
+    int a = 0;
+    while (L % 2 == 0) { L /= 2; ++a; }
+    while (a >= 3) { r.push_back(8); a -= 3; }
+    if (a == 2) r.push_back(4);
+    if (a == 1) r.push_back(2);
+    while (L % 5 == 0) { r.push_back(5); L /= 5; }
+    while (L % 3 == 0) { r.push_back(3); L /= 3; }
+    if (L != 1) throw Error(ERR_INVALID, "FFT length is not 2,3,5-smooth");
+    return r;
+}
+
+PlanRegistry& PlanRegistry::instance() {
+    static PlanRegistry r;
+    return r;
+}
+
+static constexpr int kTwCapacity = 1 << 18;
+
+FftPlan PlanRegistry::plan(int L) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = plans_.find(L);
+    if (it != plans_.end()) return it->second;
+    if (!tw64_.get()) {
+        tw64_.alloc(kTwCapacity);
+        tw32_.alloc(kTwCapacity);
+    }
+    TDDA_REQUIRE(used_ + L <= kTwCapacity, ERR_MEMORY, "twiddle pool exhausted");
+    std::vector<int> rs = radix_sequence(L);
+    TDDA_REQUIRE((int)rs.size() <= FFT_MAX_STAGES, ERR_INVALID, "too many FFT stages");
+    FftPlan p{};
+    p.L = L;
+    p.nst = (int)rs.size();
+    for (int i = 0; i < p.nst; ++i) p.radix[i] = rs[i];
+    p.tw_off = used_;
+    std::vector<double2> h64(L);
+    std::vector<float2> h32(L);
+    for (int m = 0; m < L; ++m) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)L;
+        h64[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+        h32[m] = make_float2((float)h64[m].x, (float)h64[m].y);
+    }
+    TDDA_HIP(hipMemcpy(tw64_.get() + used_, h64.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+    TDDA_HIP(hipMemcpy(tw32_.get() + used_, h32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+    used_ += L;
+    plans_[L] = p;
+    return p;
+}
+
+TwiddlePools PlanRegistry::pools() {
+    std::lock_guard<std::mutex> lk(mu_);
+    TwiddlePools t;
+    t.tw32 = tw32_.get();
+    t.tw64 = tw64_.get();
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Si/Ci tables
+// ---------------------------------------------------------------------------
+namespace {
+struct SiciEntry {
+    DevBuf<double> si, ci;
+    long n;
+    double delta;
+};
+std::mutex g_sici_mu;
+std::map<uint64_t, std::unique_ptr<SiciEntry>> g_sici;
+
+uint64_t fnv64(const void* data, size_t bytes, uint64_t h) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < bytes; ++i) { h ^= p[i]; h *= 1099511628211ull; }
+    return h;
+}
+}  // namespace
+
+SiciDevice sici_tables(const double* si, const double* ci, long n, double delta) {
+    SiciDevice out;
+    out.n = n;
+    out.delta = delta;
+    if (n <= 0 || si == nullptr || ci == nullptr) return out;   // LDR needs none
+    uint64_t h = 1469598103934665603ull;
+    h = fnv64(&n, sizeof n, h);
+    h = fnv64(&delta, sizeof delta, h);
+    h = fnv64(si, n * sizeof(double), h);
+    h = fnv64(ci, n * sizeof(double), h);
+    std::lock_guard<std::mutex> lk(g_sici_mu);
+    auto it = g_sici.find(h);
+    if (it == g_sici.end()) {
+        auto e = std::make_unique<SiciEntry>();
+        e->si.alloc(n);
+        e->ci.alloc(n);
+        TDDA_HIP(hipMemcpy(e->si.get(), si, n * sizeof(double), hipMemcpyHostToDevice));
+        TDDA_HIP(hipMemcpy(e->ci.get(), ci, n * sizeof(double), hipMemcpyHostToDevice));
+        e->n = n;
+        e->delta = delta;
+        it = g_sici.emplace(h, std::move(e)).first;
+    }
+    out.si = it->second->si.get();
+    out.ci = it->second->ci.get();
+    out.id = h;
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+bool OperatorKey::operator<(const OperatorKey& o) const {
+    return std::tie(nx, ny, nz, d, K, method, maxm, maxn, lm, ln, sici_id) <
+           std::tie(o.nx, o.ny, o.nz, o.d, o.K, o.method, o.maxm, o.maxn, o.lm, o.ln, o.sici_id);
+}
+
+Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t st) : key(k) {
+    auto t0 = std::chrono::steady_clock::now();
+    nx = k.nx; ny = k.ny; nz = k.nz;
+    TDDA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, ERR_INVALID, "grid dimensions must be positive");
+    lx = choose_fft_length(nx);
+    ly = choose_fft_length(ny);
+    lz = choose_fft_length(nz);
+    TDDA_REQUIRE(lx <= MAX_LXY && ly <= MAX_LXY && lz <= MAX_LZ, ERR_INVALID,
+                 "grid too large for one device transform (Nx,Ny <= 256, Nz <= 128)");
+    hx = lx / 2 + 1; hy = ly / 2 + 1; hz = lz / 2 + 1;
+    PlanRegistry& reg = PlanRegistry::instance();
+    px = reg.plan(lx);
+    py = reg.plan(ly);
+    pz = reg.plan(lz);
+    TwiddlePools tw = reg.pools();
+    if (k.method == 0)
+        TDDA_REQUIRE(tables.si != nullptr && tables.n > 0, ERR_INVALID, "FCD needs Si/Ci tables");
+
+    GreenParams P{};
+    P.nx = nx; P.ny = ny; P.nz = nz;
+    P.d = k.d; P.K = k.K; P.method = k.method;
+    P.maxm = k.maxm; P.maxn = k.maxn; P.lm = k.lm; P.ln = k.ln;
+    P.nsici = tables.n; P.delta = tables.delta;
+
+    const long noct = (long)nx * ny * nz;
+    DevBuf<double2> a(6 * noct);
+    DevBuf<double2> b(6L * nz * ny * hx);
+    DevBuf<double2> c(6L * nz * hy * hx);
+    green64.alloc(6L * hz * hy * hx);
+    green32.alloc(6L * hz * hy * hx);
+    launch_green_octant(P, tables.si, tables.ci, a.get(), st);
+    launch_green_dft(a.get(), b.get(), nz * ny, nx, hx, 1, lx, 0, tw.tw64 + px.tw_off, st);
+    launch_green_dft(b.get(), c.get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
+    launch_green_dft(c.get(), green64.get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
+    launch_to_c64(green64.get(), green32.get(), (long)green64.size(), st);
+    TDDA_HIP(hipStreamSynchronize(st));
+    setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+OperatorCache& OperatorCache::instance() {
+    static OperatorCache c;
+    return c;
+}
+
+size_t OperatorCache::bytes() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t s = 0;
+    for (auto& kv : map_) s += kv.second->bytes();
+    return s;
+}
+
+void OperatorCache::clear() {
+    std::lock_guard<std::mutex> lk(mu_);
+    map_.clear();
+    lru_.clear();
+}
+
+std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st,
+                                             bool* hit) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = map_.find(key);
+        if (it != map_.end()) {
+            auto pos = std::find_if(lru_.begin(), lru_.end(), [&](const OperatorKey& k) {
+                return !(k < key) && !(key < k);
+            });
+            if (pos != lru_.end()) lru_.erase(pos);
+            lru_.push_back(key);
+            if (hit) *hit = true;
+            return it->second;
+        }
+    }
+    if (hit) *hit = false;
+    auto op = std::make_shared<Operator>(key, tables, st);
+    std::lock_guard<std::mutex> lk(mu_);
+    map_[key] = op;
+    lru_.push_back(key);
+    size_t total = 0;
+    for (auto& kv : map_) total += kv.second->bytes();
+    // evict least recently used entries nobody else holds
+    for (size_t i = 0; i < lru_.size() && total > budget_;) {
+        auto it = map_.find(lru_[i]);
+        if (it != map_.end() && it->second.use_count() == 1 && it->second != op) {
+            total -= it->second->bytes();
+            map_.erase(it);
+            lru_.erase(lru_.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+    return op;
+}
+
+// ---------------------------------------------------------------------------
+// systems
+// ---------------------------------------------------------------------------
+static size_t grid_elems(const Operator& op) { return (size_t)3 * op.nz * op.ly * op.lx; }
+
+size_t System::estimate_bytes(const Operator& op, int ndip, bool refined) {
+    const size_t n3 = 3ull * ndip;
+    const size_t ks = refined ? sizeof(float2) : sizeof(double2);
+    size_t b = grid_elems(op) * sizeof(double2);
+    b += 6 * n3 * ks;
+    b += 3 * n3 * sizeof(double2);   // P, rhs, scratch64
+    b += ndip * (sizeof(double2) + sizeof(float2));
+    b += (size_t)std::max<long>(n3 / (FFT_THREADS * UPD_EPT) + 1, (long)op.ny * op.nz) * 6 * sizeof(double);
+    return b;
+}
+
+System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host, bool refined_, hipStream_t st)
+    : op(std::move(op_)), ndip(ndip_), refined(refined_) {
+    const Operator& o = *op;
+    TDDA_REQUIRE(ndip >= 1, ERR_INVALID, "system needs at least one dipole");
+    const size_t n3 = 3ull * ndip;
+    const size_t kbytes = refined ? sizeof(float2) : sizeof(double2);
+    grid.alloc(grid_elems(o) * sizeof(double2));
+    kvec.alloc(6 * n3 * kbytes);
+    P.alloc(n3);
+    rhs.alloc(n3);
+    scratch64.alloc(n3);
+    al64.alloc(ndip);
+    al32.alloc(ndip);
+    counter.alloc(1);
+    counter.zero(st);
+    P.zero(st);
+
+    std::memset(&job, 0, sizeof job);
+    job.nx = o.nx; job.ny = o.ny; job.nz = o.nz; job.ndip = ndip;
+    job.lx = o.lx; job.ly = o.ly; job.lz = o.lz;
+    job.hx = o.hx; job.hy = o.hy; job.hz = o.hz;
+    job.px = o.px; job.py = o.py; job.pz = o.pz;
+    plan_tiles(job);
+    const long nparts = std::max({(long)job.tupd, (long)job.tx32, (long)job.tx64});
+    partial.alloc(nparts * 6);
+    if (box_map_host) {
+        box_map.alloc((size_t)o.nx * o.ny * o.nz);
+        box_map.upload(box_map_host, box_map.size(), st);
+        job.box_map = box_map.get();
+    } else {
+        TDDA_REQUIRE((long)ndip == (long)o.nx * o.ny * o.nz, ERR_INVALID,
+                     "identity dipole map needs the full box");
+        job.box_map = nullptr;
+    }
+    job.green32 = o.green32.get();
+    job.green64 = o.green64.get();
+    job.al32 = al32.get();
+    job.al64 = al64.get();
+    job.grid = grid.get();
+    char* k = kvec.get();
+    job.r = k;
+    job.r0 = k + 1 * n3 * kbytes;
+    job.p = k + 2 * n3 * kbytes;
+    job.v = k + 3 * n3 * kbytes;
+    job.t = k + 4 * n3 * kbytes;
+    job.s = k + 5 * n3 * kbytes;
+    job.P = P.get();
+    job.rhs = rhs.get();
+    job.partial = partial.get();
+    job.counter = counter.get();
+    job.ks = nullptr;
+}
+
+size_t System::bytes() const {
+    return grid.bytes() + kvec.bytes() + P.bytes() + rhs.bytes() + scratch64.bytes() + al64.bytes() +
+           al32.bytes() + partial.bytes() + counter.bytes() + box_map.bytes();
+}
+
+namespace {
+__global__ void k_alpha32(const double2* __restrict__ a, float2* __restrict__ b, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+}  // namespace
+
+void System::set_alpha(const double2* al_dip_host, hipStream_t st) {
+    al64.upload(al_dip_host, ndip, st);
+    hipLaunchKernelGGL(k_alpha32, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(), (long)ndip);
+    TDDA_HIP(hipGetLastError());
+}
+
+void System::set_alpha_device(const double2* al_dip_dev, hipStream_t st) {
+    TDDA_HIP(hipMemcpyAsync(al64.get(), al_dip_dev, ndip * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_alpha32, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(), (long)ndip);
+    TDDA_HIP(hipGetLastError());
+}
+
+void System::set_rhs(const double2* h, hipStream_t st) { rhs.upload(h, 3ull * ndip, st); }
+void System::set_solution(const double2* h, hipStream_t st) { P.upload(h, 3ull * ndip, st); }
+void System::get_solution(double2* h, hipStream_t st) const { P.download(h, 3ull * ndip, st); }
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st) {
+    hjobs.clear();
+    std::vector<int2> m[7];
+    size_t lds[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < systems.size(); ++j) {
+        System* s = systems[j];
+        TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
+        GridJob J = s->job;
+        J.ks = ks_dev ? ks_dev + j : nullptr;
+        hjobs.push_back(J);
+        const int jj = (int)j;
+        for (int t = 0; t < J.tx32; ++t) m[0].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.ty32; ++t) m[1].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.tz32; ++t) m[2].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.tx64; ++t) m[3].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.ty64; ++t) m[4].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.tz64; ++t) m[5].push_back(make_int2(jj, t));
+        for (int t = 0; t < J.tupd; ++t) m[6].push_back(make_int2(jj, t));
+        lds[0] = std::max(lds[0], lds_bytes_x(J, true));
+        lds[1] = std::max(lds[1], lds_bytes_y(J, true));
+        lds[2] = std::max(lds[2], lds_bytes_z(J, true));
+        lds[3] = std::max(lds[3], lds_bytes_x(J, false));
+        lds[4] = std::max(lds[4], lds_bytes_y(J, false));
+        lds[5] = std::max(lds[5], lds_bytes_z(J, false));
+    }
+    size_t total = 0;
+    for (auto& v : m) total += v.size();
+    std::vector<int2> all;
+    all.reserve(total);
+    size_t off[7];
+    for (int i = 0; i < 7; ++i) {
+        off[i] = all.size();
+        all.insert(all.end(), m[i].begin(), m[i].end());
+    }
+    if (maps_.size() < total) maps_.alloc(std::max<size_t>(total, 1));
+    if (total) maps_.upload(all.data(), total, st);
+    if (djobs_.size() < hjobs.size()) djobs_.alloc(std::max<size_t>(hjobs.size(), 1));
+    upload_jobs(st);
+    const int2* base = maps_.get();
+    ls = LaunchSet{};
+    ls.jobs = djobs_.get();
+    ls.map_x32 = base + off[0]; ls.n_x32 = (int)m[0].size();
+    ls.map_y32 = base + off[1]; ls.n_y32 = (int)m[1].size();
+    ls.map_z32 = base + off[2]; ls.n_z32 = (int)m[2].size();
+    ls.map_x64 = base + off[3]; ls.n_x64 = (int)m[3].size();
+    ls.map_y64 = base + off[4]; ls.n_y64 = (int)m[4].size();
+    ls.map_z64 = base + off[5]; ls.n_z64 = (int)m[5].size();
+    ls.map_upd = base + off[6]; ls.n_upd = (int)m[6].size();
+    ls.lds_x32 = lds[0]; ls.lds_y32 = lds[1]; ls.lds_z32 = lds[2];
+    ls.lds_x64 = lds[3]; ls.lds_y64 = lds[4]; ls.lds_z64 = lds[5];
+    ls.refined = refined;
+}
+
+void Batch::upload_jobs(hipStream_t st) {
+    if (!hjobs.empty()) djobs_.upload(hjobs.data(), hjobs.size(), st);
+}
+
+// ---------------------------------------------------------------------------
+// solving
+// ---------------------------------------------------------------------------
+std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const SolveOptions& opt, hipStream_t st) {
+    const size_t nj = systems.size();
+    std::vector<SolveResult> res(nj);
+    if (nj == 0) return res;
+    auto t0 = std::chrono::steady_clock::now();
+    TwiddlePools tw = PlanRegistry::instance().pools();
+    DevBuf<KScalars> ks(nj);
+    std::vector<KScalars> hks(nj);
+    std::memset(hks.data(), 0, nj * sizeof(KScalars));
+    for (size_t j = 0; j < nj; ++j) {
+        hks[j].tol = opt.tol;
+        hks[j].maxit = opt.maxit;
+        hks[j].refined = opt.refined ? 1 : 0;
+        hks[j].ru_delta = opt.ru_delta;
+        hks[j].status = JOB_ACTIVE;
+    }
+    ks.upload(hks.data(), nj, st);
+    Batch b;
+    b.build(systems, ks.get(), opt.refined, st);
+    if (opt.warm) {
+        // scratch64 = A P0 + al P0 with FP64 transforms
+        for (size_t j = 0; j < nj; ++j) {
+            b.hjobs[j].xin = systems[j]->P.get();
+            b.hjobs[j].yout = systems[j]->scratch64.get();
+        }
+        b.upload_jobs(st);
+        LaunchSet l64 = b.ls;
+        l64.refined = opt.refined;
+        launch_matvec(l64, tw, false, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+    }
+    launch_init(b.ls, opt.warm, st);
+    const bool c64 = opt.refined;
+    std::vector<KScalars> snap(nj);
+    int done_iters = 0;
+    int chunk = opt.chunk > 0 ? opt.chunk : 4;
+    while (true) {
+        for (int k = 0; k < chunk; ++k) {
+            launch_matvec(b.ls, tw, c64, XF_P, XI_V, GATE_ACTIVE, st);
+            launch_matvec(b.ls, tw, c64, XF_T, XI_S, GATE_ACTIVE, st);
+            launch_update(b.ls, GATE_ACTIVE, st);
+            if (opt.refined) launch_matvec(b.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, st);
+        }
+        done_iters += chunk;
+        ks.download(snap.data(), nj, st);
+        TDDA_HIP(hipStreamSynchronize(st));
+        bool all_done = true;
+        int max_iter = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            if (snap[j].status == JOB_ACTIVE) all_done = false;
+            max_iter = std::max(max_iter, snap[j].iter);
+        }
+        if (all_done) break;
+        if (opt.chunk <= 0) {
+            // grow the chunk while the solve keeps going; bounded by maxit
+            chunk = std::min(32, chunk * 2);
+            chunk = std::max(1, std::min(chunk, opt.maxit - max_iter));
+        }
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t j = 0; j < nj; ++j) {
+        res[j].status = snap[j].status;
+        res[j].iterations = snap[j].iterations;
+        res[j].err = snap[j].err;
+        res[j].n_ru = snap[j].n_ru;
+        res[j].ms = ms;
+    }
+    return res;
+}
+
+void apply_operator64(System& s, const double2* x, double2* y, hipStream_t st) {
+    Batch b;
+    std::vector<System*> v{&s};
+    b.build(v, nullptr, false, st);
+    b.hjobs[0].xin = x;
+    b.hjobs[0].yout = y;
+    b.upload_jobs(st);
+    LaunchSet l = b.ls;
+    l.refined = s.refined;
+    launch_matvec(l, PlanRegistry::instance().pools(), false, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+    TDDA_HIP(hipStreamSynchronize(st));
+}
+
+void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st) {
+    TDDA_REQUIRE(s.refined, ERR_INVALID, "c64 operator needs a refined system");
+    Batch b;
+    std::vector<System*> v{&s};
+    b.build(v, nullptr, true, st);
+    b.hjobs[0].xin = x;
+    b.hjobs[0].yout = y;
+    b.upload_jobs(st);
+    launch_matvec(b.ls, PlanRegistry::instance().pools(), true, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+    TDDA_HIP(hipStreamSynchronize(st));
+}
+
+void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5) {
+    TDDA_REQUIRE(!c64 || s.refined, ERR_INVALID, "c64 transforms need a refined system");
+    TDDA_REQUIRE(reps >= 1, ERR_INVALID, "reps must be positive");
+    Batch b;
+    std::vector<System*> v{&s};
+    b.build(v, nullptr, s.refined, st);
+    const size_t n3 = 3ull * s.ndip;
+    DevBuf<double2> x(n3), y(n3);
+    x.zero(st);
+    b.hjobs[0].xin = x.get();
+    b.hjobs[0].yout = y.get();
+    b.upload_jobs(st);
+    for (int k = 0; k < 5; ++k) ms5[k] = 0.0;
+    LaunchSet l = b.ls;
+    launch_matvec_timed(l, PlanRegistry::instance().pools(), c64, st, 1, ms5);   // warm up
+    for (int k = 0; k < 5; ++k) ms5[k] = 0.0;
+    launch_matvec_timed(l, PlanRegistry::instance().pools(), c64, st, reps, ms5);
+    for (int k = 0; k < 5; ++k) ms5[k] /= reps;
+}
+
+}  // namespace tdda

@@ -1,0 +1,215 @@
+// Host runtime: device memory, FFT plans, interaction operators, linear systems
+// and the batched BiCGSTAB driver.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "dda_types.h"
+#include "green.h"
+#include "passes.h"
+
+namespace tdda {
+
+// ---------------------------------------------------------------------------
+// device buffer (move-only RAII)
+// ---------------------------------------------------------------------------
+size_t device_bytes_in_use();
+
+template <typename T> class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t n);
+    void release();
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+    void upload(const T* h, size_t n, hipStream_t st) {
+        TDDA_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void download(T* h, size_t n, hipStream_t st) const {
+        TDDA_HIP(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    }
+    void zero(hipStream_t st) { if (n_) TDDA_HIP(hipMemsetAsync(p_, 0, n_ * sizeof(T), st)); }
+
+   private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+
+template <typename T> void DevBuf<T>::alloc(size_t n) {
+    release();
+    if (n == 0) return;
+    p_ = static_cast<T*>(dev_alloc(n * sizeof(T)));
+    n_ = n;
+}
+template <typename T> void DevBuf<T>::release() {
+    if (p_) dev_free(p_, n_ * sizeof(T));
+    p_ = nullptr;
+    n_ = 0;
+}
+
+// ---------------------------------------------------------------------------
+// FFT lengths, plans and twiddle tables
+// ---------------------------------------------------------------------------
+constexpr int MAX_LXY = 512;   // limits set by the per-tile caps
+constexpr int MAX_LZ = 256;
+
+int choose_fft_length(int n);             // smallest 2,3,5-smooth L >= max(2, 2n-1)
+std::vector<int> radix_sequence(int L);   // forward stage radices from {8,4,2,5,3}
+
+class PlanRegistry {
+   public:
+    static PlanRegistry& instance();
+    FftPlan plan(int L);   // builds and uploads twiddles on first use (synchronous)
+    TwiddlePools pools();
+
+   private:
+    std::mutex mu_;
+    std::map<int, FftPlan> plans_;
+    DevBuf<float2> tw32_;
+    DevBuf<double2> tw64_;
+    int used_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Si/Ci tables (device copies keyed by content)
+// ---------------------------------------------------------------------------
+struct SiciDevice {
+    const double* si = nullptr;
+    const double* ci = nullptr;
+    long n = 0;
+    double delta = 0.1;
+    uint64_t id = 0;
+};
+SiciDevice sici_tables(const double* si, const double* ci, long n, double delta);
+
+// ---------------------------------------------------------------------------
+// interaction operator: octant Green's spectrum for one (grid, K, images, method)
+// ---------------------------------------------------------------------------
+struct OperatorKey {
+    int nx, ny, nz;
+    double d, K;
+    int method, maxm, maxn;
+    double lm, ln;
+    uint64_t sici_id;
+    bool operator<(const OperatorKey& o) const;
+};
+
+class Operator {
+   public:
+    Operator(const OperatorKey& key, const SiciDevice& tables, hipStream_t st);
+    OperatorKey key;
+    int nx, ny, nz, lx, ly, lz, hx, hy, hz;
+    FftPlan px, py, pz;
+    DevBuf<double2> green64;   // [6][hz][hy][hx]
+    DevBuf<float2> green32;
+    double setup_ms = 0.0;
+    size_t bytes() const { return green64.bytes() + green32.bytes(); }
+};
+
+// LRU cache of operators bounded by bytes
+class OperatorCache {
+   public:
+    static OperatorCache& instance();
+    std::shared_ptr<Operator> get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st, bool* hit);
+    void set_budget(size_t bytes) { budget_ = bytes; }
+    size_t bytes() const;
+    void clear();
+
+   private:
+    mutable std::mutex mu_;
+    std::map<OperatorKey, std::shared_ptr<Operator>> map_;
+    std::vector<OperatorKey> lru_;
+    size_t budget_ = size_t(8) << 30;
+};
+
+// ---------------------------------------------------------------------------
+// one linear system (diag(al) + A) P = rhs and its device memory
+// ---------------------------------------------------------------------------
+class System {
+   public:
+    // box_map: [nz][ny][nx] -> dipole index or -1; null means the full box in
+    // x-fastest order (python_bindings/dda_model/__init__.py:8-20).
+    System(std::shared_ptr<Operator> op, int ndip, const int* box_map_host, bool refined, hipStream_t st);
+    void set_alpha(const double2* al_dip_host, hipStream_t st);   // 1/alpha per dipole
+    void set_alpha_device(const double2* al_dip_dev, hipStream_t st);
+    void set_rhs(const double2* rhs_host, hipStream_t st);
+    void set_solution(const double2* P_host, hipStream_t st);
+    void get_solution(double2* P_host, hipStream_t st) const;
+    size_t bytes() const;
+    static size_t estimate_bytes(const Operator& op, int ndip, bool refined);
+
+    std::shared_ptr<Operator> op;
+    int ndip;
+    bool refined;
+    GridJob job;   // host copy of the descriptor (ks set by the batch)
+    DevBuf<char> grid;          // c128-sized work grid (c64 view aliases it)
+    DevBuf<char> kvec;          // r, r0, p, v, t, s
+    DevBuf<double2> P, rhs, scratch64;
+    DevBuf<double2> al64;
+    DevBuf<float2> al32;
+    DevBuf<double> partial;
+    DevBuf<unsigned> counter;
+    DevBuf<int> box_map;
+};
+
+struct SolveOptions {
+    bool refined = false;   // c64 Krylov + FP64 reliable updates (requires refined systems)
+    int maxit = 100000;
+    double tol = 1e-5;
+    double ru_delta = 0.1;
+    bool warm = false;      // use the system's current P as the initial guess
+    int chunk = 0;          // iterations between host checks (0: adaptive)
+};
+
+struct SolveResult {
+    int status = JOB_IDLE;
+    int iterations = 0;
+    double err = 0.0;
+    int n_ru = 0;
+    double ms = 0.0;
+};
+
+// Device-resident batch: descriptors and (job, tile) maps for every pass.
+class Batch {
+   public:
+    void build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st);
+    void upload_jobs(hipStream_t st);   // after editing hjobs (xin/yout)
+    LaunchSet ls;
+    std::vector<GridJob> hjobs;
+
+   private:
+    DevBuf<GridJob> djobs_;
+    DevBuf<int2> maps_;
+};
+
+// Solve every system in the batch with BiCGSTAB (DDAModel.cpp:899-1009).
+std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const SolveOptions& opt, hipStream_t st);
+
+// y = A x + al x with FP64 transforms (x, y device, 3*ndip)
+void apply_operator64(System& s, const double2* x, double2* y, hipStream_t st);
+// same with c64 transforms (requires a refined system)
+void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st);
+
+// Average ms per launch of the five passes (X fwd, Y fwd, Z, Y inv, X inv)
+// of y = A x + al x over `reps` repetitions, c64 or c128 transforms.
+void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5);
+
+hipStream_t default_stream();
+
+}  // namespace tdda
