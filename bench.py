@@ -1,20 +1,24 @@
-"""Benchmark of the DDA solve path on one MI355X.
+"""Benchmark of the DDA solve path on one MI355X; prints ONE JSON line.
 
-Prints ONE JSON line.  A step is one solve of the forward system to a 1e-5
-relative residual, with the operator, 1/alpha and the right-hand side already
-resident on the device.
+Workload (headline): a stream of R DDA requests (default 512, seeds 1000..),
+each independently drawn: box uniform over {16^3, 24^3, 32x32x16, 48x48x24,
+64x64x32} with d = 15 nm, wavelength uniform in [400, 800] nm, polar angle
+uniform in [0, 30 deg], azimuth and polarisation angle uniform, density
+uniform in [0, 1) per voxel, LDR interaction, non-periodic.  Every request is
+solved to a 1e-5 relative residual (true FP64 residual) by the continuous-
+batching server in complex64 with FP64 reliable updates.
 
-Workload: 2-D periodic unit cell, 32x32x12 dipoles, d = 10 nm, periods 320 nm,
-FCD interaction with a +-50 image lattice sum in FP64, lambda = 600 nm, normal
-incidence, y polarisation, smooth seeded density (Gaussian filter radius 2,
-clipped to [0, 1]).  `value` is the FP32-refined solver's solves/s; the plain
-FP64 solver is timed on the same system in the same run.
+A step is one pass over the R requests with all of them queued at once (the
+server's saturation throughput); `value` is solves/s over the K timed steps.
+The same requests then arrive as a Poisson stream at 0.8x that rate, which
+gives the p50/p99 latency.  The plain FP64 single-solve GPU path (one request
+at a time, FP64 transforms and Krylov vectors) is timed on a subset of the
+same requests for the speedup, and the FP64 CPU oracle on a bounded sample.
 """
 import argparse
 import json
 import math
 import os
-import statistics
 import sys
 import time
 
@@ -26,139 +30,203 @@ for p in (os.path.join(ROOT, "topo-dda_amd"), os.path.join(ROOT, "oracle")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 16), (48, 48, 24), (64, 64, 32)]
 
 
-def smooth_density(shape_xyz, sigma, seed):
-    """rho = clip(0.5 + 0.25 f / std(f), 0, 1), f = Gaussian-filtered N(0,1)
-    noise on (z, y, x), periodic in x and y, nearest at the z faces."""
-    from scipy.ndimage import gaussian_filter
-    nx, ny, nz = shape_xyz
-    rng = np.random.default_rng(seed)
-    f = gaussian_filter(rng.standard_normal((nz, ny, nx)), sigma=sigma, mode=("nearest", "wrap", "wrap"))
-    rho = np.clip(0.5 + 0.25 * f / f.std(), 0.0, 1.0)
-    return rho.reshape(-1)
+def fft_len(n):
+    L = max(2, 2 * n - 1)
+    while True:
+        m = L
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return L
+        L += 1
 
 
-def unit_cell_args(tables):
-    """Positional DDAModel arguments of the 32x32x12 periodic unit cell."""
-    nx, ny, nz = 32, 32, 12
-    rho = smooth_density((nx, ny, nz), 2.0, seed=1)
-    import dda_model
-    geometry = dda_model._generate_geometry(nx, ny, nz)
-    si, ci, delta = tables
-    n = nx * ny * nz
-    return [0.0, 50.0, 0.5, "piecewise", [100], [2.0], False,
-            "None", [], True, "IntegratedE", [2, 0.0, nx - 1.0, 0.0, ny - 1.0, 0.0, nz - 1.0, 0.95, 50.0],
-            geometry, np.repeat(rho, 3), nx, ny, nz, n,
-            [0.0, 0.0, 1.0], 1.0, [0.0, 1.0, 0.0], 600.0, [1.0 + 0j, 5.96 + 3.89j], 1.0,
-            50, 50, 32.0, 32.0, "FCD", 10.0, si, ci, delta, False]
+def make_requests(native, count, seed0=1000):
+    reqs = []
+    for i in range(count):
+        rng = np.random.default_rng(seed0 + i)
+        shape = SHAPES[int(rng.integers(len(SHAPES)))]
+        lam = float(rng.uniform(400.0, 800.0))
+        th = float(rng.uniform(0.0, math.radians(30.0)))
+        ph = float(rng.uniform(0.0, 2 * math.pi))
+        ps = float(rng.uniform(0.0, 2 * math.pi))
+        k = (math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th))
+        e1 = (math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th))
+        e2 = (-math.sin(ph), math.cos(ph), 0.0)
+        e = tuple(math.cos(ps) * a + math.sin(ps) * b for a, b in zip(e1, e2))
+        rho = rng.uniform(0.0, 1.0, int(np.prod(shape)))
+        reqs.append(native.Request(shape=shape, d=15.0, lam=lam, rho=rho, n_k=k, n_e0=e, method="LDR"))
+    return reqs
 
 
 def percentile(v, q):
     s = sorted(v)
+    if not s:
+        return float("nan")
     k = (len(s) - 1) * q
     lo, hi = math.floor(k), math.ceil(k)
     return s[lo] + (s[hi] - s[lo]) * (k - lo)
 
 
-def cpu_baseline(args, tol):
-    """The FP64 oracle (numpy + scipy.fft, one worker) on the same system:
-    Green's setup timed apart, then one full solve."""
+def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=False, timeout=600.0):
+    """Submit every request (at its arrival time if given) and wait for all.
+    Returns (wall seconds from first submission to last completion, results)."""
+    srv = native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields)
+    results = {}
+    t0 = time.perf_counter()
+    nxt = 0
+    deadline = t0 + timeout
+    while len(results) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and (arrivals is None or arrivals[nxt] <= now):
+            srv.submit(nxt, reqs[nxt])
+            nxt += 1
+        for r in srv.poll():
+            results[r["id"]] = r
+        if time.perf_counter() > deadline:
+            raise RuntimeError(f"stream did not finish in {timeout} s ({len(results)}/{len(reqs)})")
+        time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    out_fields = {i: srv.field(i) for i in results} if fields else None
+    stats = srv.stats()
+    srv.close()
+    return wall, results, stats, out_fields
+
+
+def cpu_baseline(reqs, budget_s=20.0):
+    """FP64 oracle (numpy + scipy.fft, one worker) on the first requests of the
+    stream until about budget_s of CPU work."""
     import dda_oracle as O
-    t0 = time.perf_counter()
-    m = O.OracleDDAModel(*args, workers=1)
-    setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    m.solveElectricField(np.zeros(3 * m.N, complex), 100000, tol)
-    solve = time.perf_counter() - t0
-    return setup, solve, m
+    done, solve_s, setup_s, iters = 0, 0.0, 0.0, []
+    for r in reqs:
+        a = O.wrapper_args(list(r.shape), r.d, np.repeat(r.rho, 3), list(r.material), list(r.n_k), list(r.n_e0),
+                           r.lam, symmetry_method="None", apc_max_m=0, apc_max_n=0, apc_method_name="LDR",
+                           tables=(np.zeros(2), np.zeros(2), 0.1))
+        t0 = time.perf_counter()
+        m = O.OracleDDAModel(*a, workers=1)
+        t1 = time.perf_counter()
+        m.solveElectricField(np.zeros(3 * m.N, complex), 100000, 1e-5)
+        t2 = time.perf_counter()
+        setup_s += t1 - t0
+        solve_s += t2 - t1
+        iters.append(m.info.iterations)
+        done += 1
+        if setup_s + solve_s > budget_s:
+            break
+    return done, setup_s, solve_s, iters
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--fp64-requests", type=int, default=96)
+    ap.add_argument("--slots", type=int, default=128)
+    ap.add_argument("--chunk", type=int, default=4)
+    ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (no multi-GPU path is built)",
               file=sys.stderr)
         sys.exit(2)
+    from dda_model import native
 
-    import dda_model
-    tables = dda_model.sici_tables()
-    args = unit_cell_args(tables)
-    t0 = time.perf_counter()
-    model = dda_model.DDAModel(*args)
-    build_s = time.perf_counter() - t0
-    (lx, ly, lz), green_ms = model.fftGrid()
-    n = 3 * 32 * 32 * 12
-    M = lx * ly * lz
+    reqs = make_requests(native, a.requests)
+    print(f"# {len(reqs)} requests generated", file=sys.stderr, flush=True)
 
-    def run(precision, steps, warmup):
-        model.setSolverPrecision(precision)
-        for _ in range(warmup):
-            model.solveResident(100000, a.tol)
-        times, infos = [], []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            s0 = time.perf_counter()
-            infos.append(model.solveResident(100000, a.tol))
-            times.append((time.perf_counter() - s0) * 1e3)
-        wall = time.perf_counter() - t0
-        return wall, times, infos
+    # warmup: a short stream (plans, twiddles, allocator, code objects)
+    for _ in range(a.warmup):
+        run_stream(native, reqs[:min(48, len(reqs))], "fp32_refined", a.slots, a.chunk)
+    print("# warmup done", file=sys.stderr, flush=True)
 
-    wall32, t32, i32 = run("fp32_refined", a.steps, a.warmup)
-    wall64, t64, i64 = run("fp64", a.steps, a.warmup)
+    # timed steps: saturation throughput of the refined batched server
+    walls, res_sat, stats_sat = [], None, None
+    for _ in range(a.steps):
+        w, res_sat, stats_sat, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk)
+        walls.append(w)
+        print(f"# step {len(walls)}: {len(reqs) / w:.1f} solves/s", file=sys.stderr, flush=True)
+    total = sum(walls)
+    value = a.steps * len(reqs) / total
+    conv = sum(1 for r in res_sat.values() if r["status"] == "converged")
 
-    # per-pass timing (HIP events, library stream)
-    model.setSolverPrecision("fp32_refined")
-    p32 = model.timePasses(50)
-    model.setSolverPrecision("fp64")
-    p64 = model.timePasses(50)
-    s = 8
-    b_z = s * (4 * M + 6 * M)                      # Z pass share of B_mv, R_op = 1
-    b_mv = s * (2 * n + 12 * M) + s * 6 * M        # per matvec
-    z_ms = p32[2]
-    roof = {"bound": "hbm", "kernel": "k_zpass<float> (z FFT + Green multiply + inverse z FFT)",
-            "achieved": round(b_z / (z_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b_z / (z_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-            "matvec_achieved_GBs": round(b_mv / (sum(p32) * 1e-3) / 1e9, 1),
-            "pass_ms_c64": [round(x, 4) for x in p32], "pass_ms_c128": [round(x, 4) for x in p64]}
+    # Poisson arrivals at load x saturation -> latency
+    rate = a.load * value
+    rng = np.random.default_rng(2)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    arrivals -= arrivals[0]
+    w_p, res_p, stats_p, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk, arrivals=arrivals)
+    lat = [(r["t_done"] - r["t_submit"]) * 1e3 for r in res_p.values()]
+    print(f"# poisson stream done: p50 {percentile(lat, 0.5):.2f} ms", file=sys.stderr, flush=True)
+
+    # plain FP64 single-solve GPU path on a subset, plus field agreement
+    sub = reqs[:min(a.fp64_requests, len(reqs))]
+    w64, res64, _, f64 = run_stream(native, sub, "fp64", 1, 8, fields=True)
+    _, res32s, _, f32 = run_stream(native, sub, "fp32_refined", a.slots, a.chunk, fields=True)
+    fp64_rate = len(sub) / w64
+    e_err = max(float(np.linalg.norm(f32[i] - f64[i]) / np.linalg.norm(f64[i])) for i in f64)
+    print(f"# fp64 single-solve: {fp64_rate:.2f} solves/s", file=sys.stderr, flush=True)
+
+    # roofline: one batched launch per pass over one problem of every shape
+    probs = [native.Problem(native.Request(shape=s, d=15.0, lam=600.0, rho=np.full(int(np.prod(s)), 0.5)),
+                            precision="fp32_refined") for s in SHAPES]
+    ms5 = native.time_passes(probs, c64=True, reps=30)
+    Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in SHAPES]
+    ns = [3 * int(np.prod(s)) for s in SHAPES]
+    b_z = sum(8 * (4 * M + 6 * M) for M in Ms)
+    b_mv = sum(8 * (2 * n + 12 * M) + 8 * 6 * M for n, M in zip(ns, Ms))
+    roof = {"bound": "hbm", "kernel": "k_zpass<float>: z FFT + Green multiply + inverse z FFT, one launch "
+                                      "over a 5-request batch (one per shape)",
+            "achieved": round(b_z / (ms5[2] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b_z / (ms5[2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": b_z, "z_ms": round(float(ms5[2]), 5),
+            "matvec_ms": round(float(sum(ms5)), 5),
+            "matvec_achieved_GBs": round(b_mv / (sum(ms5) * 1e-3) / 1e9, 1),
+            "pass_ms": [round(float(x), 5) for x in ms5]}
+    del probs
 
     cpu = None
     if not a.no_cpu:
-        setup, solve, om = cpu_baseline(args, a.tol)
-        cpu = {"value": round(1.0 / solve, 4), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"one FP64 oracle solve of the same system (numpy+scipy.fft, workers=1, "
-                         f"{om.info.iterations} iterations, {solve:.1f} s; Green's setup {setup:.1f} s apart)"}
+        n_done, setup_s, solve_s, iters = cpu_baseline(reqs)
+        cpu = {"value": round(n_done / (setup_s + solve_s), 4), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"first {n_done} requests of the stream solved by the FP64 oracle (numpy+scipy.fft, "
+                         f"workers=1): setup {setup_s:.1f} s, solves {solve_s:.1f} s, iterations {iters}"}
 
-    value = a.steps / wall32
+    iters = [r["iterations"] for r in res_sat.values()]
     out = {
-        "metric": "DDA solves/sec at 1e-5 residual, FP32-refined vs FP64; p50/p99 ms",
+        "metric": "DDA solves/sec at 1e-5 residual, FP32-refined batched vs FP64; p50/p99 ms",
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(wall32 / a.steps * 1e3, 3),
+        "ms_per_step": round(total / a.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64+f64-refine",
         "data": "synthetic",
-        "config": {"workload": "periodic unit cell 32x32x12, d=10nm, FCD +-50 images, 600nm y-pol, "
-                               "smooth density seed 1; one solve per step to 1e-5",
-                   "fft_grid": [lx, ly, lz], "green_setup_ms": round(green_ms, 2), "model_build_s": round(build_s, 3)},
-        "p50_ms": round(percentile(t32, 0.5), 3),
-        "p99_ms": round(percentile(t32, 0.99), 3),
-        "fp64": {"solves_per_s": round(a.steps / wall64, 3), "p50_ms": round(percentile(t64, 0.5), 3),
-                 "p99_ms": round(percentile(t64, 0.99), 3), "iterations": i64[-1]["iterations"]},
-        "refined": {"iterations": i32[-1]["iterations"], "reliable_updates": i32[-1]["reliable_updates"],
-                    "error": i32[-1]["error"]},
-        "speedup_vs_fp64": round((a.steps / wall32) / (a.steps / wall64), 3),
+        "config": {"workload": f"serving stream of {len(reqs)} DDA requests (boxes 16^3..64x64x32, d=15nm, "
+                               "lambda U[400,800]nm, polar angle U[0,30deg], random polarisation, density "
+                               "U[0,1), LDR, non-periodic), solved to 1e-5 true residual",
+                   "slots": a.slots, "chunk": a.chunk, "poisson_load": a.load},
+        "p50_ms": round(percentile(lat, 0.5), 3),
+        "p99_ms": round(percentile(lat, 0.99), 3),
+        "poisson_rate_per_s": round(rate, 2),
+        "poisson_solves_per_s": round(len(reqs) / w_p, 3),
+        "converged": conv,
+        "iterations_p50": percentile(iters, 0.5),
+        "fp64_single_solve": {"solves_per_s": round(fp64_rate, 3), "requests": len(sub),
+                              "iterations_p50": percentile([r["iterations"] for r in res64.values()], 0.5)},
+        "speedup_vs_fp64_single": round(value / fp64_rate, 3),
+        "field_rel_err_refined_vs_fp64_max": e_err,
+        "server": {k: stats_sat[k] for k in ("chunks", "rebuilds", "launches", "max_active", "peak_bytes")},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -139,6 +139,82 @@ int tdda_model_time_passes(tdda_model* m, int reps, double* ms5);
 /* Bytes of device memory currently held by the library. */
 long tdda_device_bytes(void);
 
+/* ------------------------------------------------------------------------
+ * Device-resident problems, batched solves and the request server.
+ * No reference counterpart: the reference solves one model at a time.
+ * A problem is a full Nx*Ny*Nz box (x-fastest dipoles, _generate_geometry of
+ * python_bindings/dda_model/__init__.py:8-20) with one density per voxel.
+ * ------------------------------------------------------------------------ */
+typedef struct tdda_problem tdda_problem;
+typedef struct tdda_server tdda_server;
+
+typedef struct {
+    int nx, ny, nz;
+    double d;          /* nm */
+    double lam;        /* nm */
+    double nback;
+    int method;        /* 0 FCD, 1 LDR */
+    int maxm, maxn;    /* image lattice range (0: non-periodic) */
+    double lm, ln;     /* image periods in cells */
+    double n_k[3];
+    double n_e0[3];
+    double e0;
+    double material[4]; /* background, foreground permittivity (re, im) */
+} tdda_problem_desc;
+
+/* precision: TDDA_PREC_*; tables may be null for LDR */
+int tdda_problem_create(const tdda_problem_desc* d, int precision, const double* si, const double* ci, long nsici,
+                        double delta, tdda_problem** out);
+int tdda_problem_destroy(tdda_problem* p);
+/* rho: nx*ny*nz densities (clipped to [0,1]); on_device: rho is a device pointer */
+int tdda_problem_set_density(tdda_problem* p, const double* rho, int on_device);
+/* solve all problems together (one batched launch per pass); infos: n entries */
+int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_solve_info* infos);
+/* E = al P 3/(eps+2): 3N complex, to host or device memory */
+int tdda_problem_field(tdda_problem* p, double* E, int on_device);
+/* polarisation P: 3N complex, to host or device memory */
+int tdda_problem_polarization(tdda_problem* p, double* P, int on_device);
+/* Average ms per launch of the five matvec passes over all n problems in one
+ * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
+ * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */
+int tdda_problems_time_passes(tdda_problem** ps, int n, int c64, int reps, double* ms5);
+
+typedef struct {
+    int precision;       /* TDDA_PREC_* */
+    int maxit;
+    double tol;
+    double ru_delta;     /* reliable-update trigger (refined precision) */
+    double hbm_budget_gb;
+    int max_slots;       /* requests in flight */
+    int chunk;           /* iterations between host checks */
+    int keep_fields;     /* keep E of finished requests for tdda_server_take_field */
+} tdda_server_opts;
+
+typedef struct {
+    long id;
+    int status;          /* TDDA_STATUS_* */
+    int iterations;
+    int reliable_updates;
+    double error;
+    double t_submit, t_admit, t_done;  /* seconds since the server started */
+    int operator_cached;
+} tdda_result;
+
+typedef struct {
+    long submitted, completed, chunks, rebuilds, launches;
+    int max_active;
+    long peak_bytes;
+} tdda_server_stats;
+
+int tdda_server_create(const tdda_server_opts* o, const double* si, const double* ci, long nsici, double delta,
+                       tdda_server** out);
+int tdda_server_destroy(tdda_server* s);
+int tdda_server_submit(tdda_server* s, long id, const tdda_problem_desc* d, const double* rho, long n);
+int tdda_server_poll(tdda_server* s, tdda_result* out, int max, int* n_out);
+int tdda_server_take_field(tdda_server* s, long id, double* E, long n);
+int tdda_server_idle(tdda_server* s, int* idle);
+int tdda_server_get_stats(tdda_server* s, tdda_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

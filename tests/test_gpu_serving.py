@@ -1,0 +1,143 @@
+"""GPU tests of device-resident problems, batched solves and the request server.
+
+Bar: FP64 fields against the oracle to 1e-8 (tight solves); a request solved
+in a heterogeneous batch or in the server mix gives the same field as the
+same request solved alone (bitwise: per-request tiling and fixed-order
+reductions make the arithmetic independent of the mix; the assertion allows
+1e-14 relative).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _request(native, shape, seed, lam=542.0, method="LDR", images=0, theta=0.0, phi=0.0, psi=0.0):
+    rng = np.random.default_rng(seed)
+    k = (math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta))
+    e1 = (math.cos(theta) * math.cos(phi), math.cos(theta) * math.sin(phi), -math.sin(theta))
+    e2 = (-math.sin(phi), math.cos(phi), 0.0)
+    e = tuple(math.cos(psi) * a + math.sin(psi) * b for a, b in zip(e1, e2))
+    return native.Request(shape=shape, d=15.0, lam=lam, rho=rng.uniform(0, 1, int(np.prod(shape))), n_k=k,
+                          n_e0=e, method=method, images=images, period=(shape[0], shape[1]))
+
+
+def _oracle_field(O, req, tables, tol=1e-12):
+    nx, ny, nz = req.shape
+    a = O.wrapper_args(list(req.shape), req.d, np.repeat(req.rho, 3), list(req.material), list(req.n_k),
+                       list(req.n_e0), req.lam, symmetry_method="None", apc_max_m=req.images,
+                       apc_max_n=req.images, apc_l_m=req.period[1] or None, apc_l_n=req.period[0] or None,
+                       apc_method_name=req.method, tables=tables)
+    m = O.OracleDDAModel(*a)
+    m.solveElectricField(np.zeros(3 * m.N, complex), 100000, tol)
+    return m.getElectricField()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def native():
+    from dda_model import native
+    return native
+
+
+def test_problem_field_matches_oracle(native, tables):
+    import dda_oracle as O
+    _log("[gpu] problem vs oracle")
+    req = _request(native, (12, 10, 6), seed=3, theta=0.3, phi=1.0, psi=0.4)
+    p = native.Problem(req, precision="fp64", tables=tables)
+    info = native.solve([p], tol=1e-12)[0]
+    assert info["status"] == "converged"
+    assert _rel(p.field(), _oracle_field(O, req, tables)) < 1e-8
+
+
+def test_problem_fcd_periodic_matches_oracle(native, tables):
+    import dda_oracle as O
+    _log("[gpu] periodic FCD problem vs oracle")
+    req = _request(native, (8, 8, 4), seed=4, lam=600.0, method="FCD", images=4)
+    p = native.Problem(req, precision="fp64", tables=tables)
+    native.solve([p], tol=1e-12)
+    assert _rel(p.field(), _oracle_field(O, req, tables)) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_batch_equals_alone(native, tables, precision):
+    _log(f"[gpu] batch vs alone {precision}")
+    reqs = [_request(native, s, seed=10 + i, lam=450 + 40 * i, theta=0.2 * i)
+            for i, s in enumerate([(16, 16, 16), (24, 24, 24), (12, 20, 8)])]
+    alone = []
+    for r in reqs:
+        p = native.Problem(r, precision=precision, tables=tables)
+        native.solve([p])
+        alone.append(p.field())
+    ps = [native.Problem(r, precision=precision, tables=tables) for r in reqs]
+    infos = native.solve(ps)
+    for p, a, inf in zip(ps, alone, infos):
+        assert inf["status"] == "converged"
+        assert _rel(p.field(), a) <= 1e-14
+
+
+def test_server_mix_equals_alone_and_oracle(native, tables):
+    import dda_oracle as O
+    _log("[gpu] server mix vs alone")
+    shapes = [(16, 16, 16), (24, 24, 24), (32, 32, 16), (16, 16, 16), (20, 12, 10), (24, 24, 24)]
+    reqs = [_request(native, s, seed=100 + i, lam=400 + 60 * i, theta=0.1 * i, phi=0.5 * i, psi=0.3 * i)
+            for i, s in enumerate(shapes)]
+    srv = native.Server(precision="fp32_refined", max_slots=8, chunk=3)
+    for i, r in enumerate(reqs):
+        srv.submit(i, r)
+    got = {}
+    for _ in range(6000):
+        for res in srv.poll():
+            got[res["id"]] = res
+        if len(got) == len(reqs):
+            break
+        import time
+        time.sleep(0.005)
+    assert len(got) == len(reqs)
+    mix = {i: srv.field(i) for i in got}
+    st = srv.stats()
+    srv.close()
+    _log(f"[gpu] server stats {st}")
+    assert st["max_active"] >= 2
+    for i, r in enumerate(reqs[:3]):
+        s1 = native.Server(precision="fp32_refined", max_slots=1, chunk=5)
+        s1.submit(0, r)
+        res = []
+        while not res:
+            res = s1.poll()
+        alone = s1.field(0)
+        s1.close()
+        assert got[i]["status"] == "converged" and res[0]["iterations"] == got[i]["iterations"]
+        assert _rel(mix[i], alone) <= 1e-14
+    # accuracy of a served result against the oracle
+    e_ref = _oracle_field(O, reqs[0], tables)
+    assert _rel(mix[0], e_ref) < 1e-4
+
+
+def test_server_admission_by_memory(native, tables):
+    """A budget below two requests' footprint keeps one request in flight; all finish."""
+    _log("[gpu] server memory admission")
+    reqs = [_request(native, (24, 24, 24), seed=200 + i) for i in range(3)]
+    srv = native.Server(precision="fp32_refined", max_slots=8, hbm_budget_gb=0.06, chunk=4)
+    for i, r in enumerate(reqs):
+        srv.submit(i, r)
+    done = []
+    for _ in range(20000):
+        done += srv.poll()
+        if len(done) == 3:
+            break
+    st = srv.stats()
+    srv.close()
+    assert len(done) == 3 and all(d["status"] == "converged" for d in done)
+    assert st["max_active"] == 1

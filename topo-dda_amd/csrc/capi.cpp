@@ -4,6 +4,7 @@
 
 #include "../../include/topodda.h"
 #include "model.h"
+#include "serve.h"
 
 namespace tdda {
 static thread_local std::string g_last_error;
@@ -248,6 +249,221 @@ int tdda_model_grid(tdda_model* m, int* lxyz, double* setup_ms) {
             lxyz[2] = m->m->op().lz;
         }
         if (setup_ms) *setup_ms = m->m->operator_setup_ms();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// problems and server
+// ---------------------------------------------------------------------------
+struct tdda_problem {
+    tdda::Problem* p;
+};
+struct tdda_server {
+    tdda::Server* s;
+};
+
+static tdda::ProblemDesc to_desc(const tdda_problem_desc* d) {
+    TDDA_REQUIRE(d != nullptr, tdda::ERR_INVALID, "null problem description");
+    tdda::ProblemDesc o{};
+    o.nx = d->nx; o.ny = d->ny; o.nz = d->nz;
+    o.d = d->d; o.lam = d->lam; o.nback = d->nback;
+    o.method = d->method; o.maxm = d->maxm; o.maxn = d->maxn; o.lm = d->lm; o.ln = d->ln;
+    for (int k = 0; k < 3; ++k) { o.nk[k] = d->n_k[k]; o.ne0[k] = d->n_e0[k]; }
+    o.e0 = d->e0;
+    for (int k = 0; k < 4; ++k) o.mat[k] = d->material[k];
+    return o;
+}
+
+int tdda_problem_create(const tdda_problem_desc* d, int precision, const double* si, const double* ci, long nsici,
+                        double delta, tdda_problem** out) {
+    return guarded([&] {
+        TDDA_REQUIRE(out != nullptr, tdda::ERR_INVALID, "null output");
+        TDDA_REQUIRE(precision == TDDA_PREC_FP64 || precision == TDDA_PREC_FP32_REFINED, tdda::ERR_INVALID,
+                     "unknown precision");
+        tdda::ProblemDesc pd = to_desc(d);
+        tdda::SiciDevice t = tdda::sici_tables(pd.method == 0 ? si : nullptr, pd.method == 0 ? ci : nullptr,
+                                               pd.method == 0 ? nsici : 0, delta);
+        auto* h = new tdda_problem{nullptr};
+        try {
+            h->p = new tdda::Problem(pd, t, precision == TDDA_PREC_FP32_REFINED, tdda::default_stream());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int tdda_problem_destroy(tdda_problem* p) {
+    return guarded([&] {
+        if (!p) return;
+        TDDA_HIP(hipDeviceSynchronize());
+        delete p->p;
+        delete p;
+    });
+}
+
+int tdda_problem_set_density(tdda_problem* p, const double* rho, int on_device) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p && rho, tdda::ERR_INVALID, "null argument");
+        p->p->set_density(rho, on_device != 0, tdda::default_stream());
+    });
+}
+
+int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_solve_info* infos) {
+    return guarded([&] {
+        TDDA_REQUIRE(ps != nullptr && n >= 1, tdda::ERR_INVALID, "no problems");
+        std::vector<tdda::System*> sys;
+        bool refined = ps[0]->p->sys->refined;
+        for (int i = 0; i < n; ++i) {
+            TDDA_REQUIRE(ps[i] && ps[i]->p, tdda::ERR_INVALID, "null problem");
+            TDDA_REQUIRE(ps[i]->p->sys->refined == refined, tdda::ERR_INVALID, "mixed precisions in one batch");
+            sys.push_back(ps[i]->p->sys.get());
+            ps[i]->p->sys->P.zero(tdda::default_stream());
+        }
+        tdda::SolveOptions o;
+        o.refined = refined;
+        o.maxit = maxit;
+        o.tol = tol;
+        std::vector<tdda::SolveResult> r = tdda::solve_batch(sys, o, tdda::default_stream());
+        if (infos)
+            for (int i = 0; i < n; ++i) {
+                infos[i].status = r[i].status;
+                infos[i].iterations = r[i].iterations;
+                infos[i].error = r[i].err;
+                infos[i].reliable_updates = r[i].n_ru;
+                infos[i].ms = r[i].ms;
+            }
+    });
+}
+
+int tdda_problem_field(tdda_problem* p, double* E, int on_device) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p && E, tdda::ERR_INVALID, "null argument");
+        hipStream_t st = tdda::default_stream();
+        const size_t n3 = 3ull * p->p->ndip;
+        if (on_device) {
+            p->p->field(reinterpret_cast<double2*>(E), st);
+        } else {
+            tdda::DevBuf<double2> b(n3);
+            p->p->field(b.get(), st);
+            b.download(reinterpret_cast<double2*>(E), n3, st);
+        }
+        TDDA_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int tdda_problem_polarization(tdda_problem* p, double* P, int on_device) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p && P, tdda::ERR_INVALID, "null argument");
+        const size_t n3 = 3ull * p->p->ndip;
+        TDDA_HIP(hipMemcpy(P, p->p->sys->P.get(), n3 * sizeof(double2),
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    });
+}
+
+int tdda_problems_time_passes(tdda_problem** ps, int n, int c64, int reps, double* ms5) {
+    return guarded([&] {
+        TDDA_REQUIRE(ps != nullptr && n >= 1 && ms5 != nullptr, tdda::ERR_INVALID, "no problems");
+        std::vector<tdda::System*> sys;
+        for (int i = 0; i < n; ++i) {
+            TDDA_REQUIRE(ps[i] && ps[i]->p, tdda::ERR_INVALID, "null problem");
+            sys.push_back(ps[i]->p->sys.get());
+        }
+        tdda::time_passes_batch(sys, c64 != 0, reps, tdda::default_stream(), ms5);
+    });
+}
+
+int tdda_server_create(const tdda_server_opts* o, const double* si, const double* ci, long nsici, double delta,
+                       tdda_server** out) {
+    return guarded([&] {
+        TDDA_REQUIRE(o && out, tdda::ERR_INVALID, "null argument");
+        tdda::ServerOptions so;
+        so.refined = o->precision == TDDA_PREC_FP32_REFINED;
+        so.maxit = o->maxit;
+        so.tol = o->tol;
+        so.ru_delta = o->ru_delta > 0 ? o->ru_delta : 0.1;
+        so.hbm_budget = (size_t)(o->hbm_budget_gb * (double)(1ull << 30));
+        so.max_slots = o->max_slots;
+        so.chunk = o->chunk;
+        so.keep_fields = o->keep_fields != 0;
+        tdda::SiciDevice t = tdda::sici_tables(si, ci, si ? nsici : 0, delta);
+        *out = new tdda_server{new tdda::Server(so, t)};
+    });
+}
+
+int tdda_server_destroy(tdda_server* s) {
+    return guarded([&] {
+        if (!s) return;
+        delete s->s;
+        delete s;
+    });
+}
+
+int tdda_server_submit(tdda_server* s, long id, const tdda_problem_desc* d, const double* rho, long n) {
+    return guarded([&] {
+        TDDA_REQUIRE(s && s->s && rho, tdda::ERR_INVALID, "null argument");
+        tdda::ServeRequest r;
+        r.id = id;
+        r.desc = to_desc(d);
+        TDDA_REQUIRE(r.desc.method == 0 || r.desc.method == 1, tdda::ERR_METHOD, "method must be FCD or LDR");
+        r.rho.assign(rho, rho + n);
+        s->s->submit(std::move(r));
+    });
+}
+
+int tdda_server_poll(tdda_server* s, tdda_result* out, int max, int* n_out) {
+    return guarded([&] {
+        TDDA_REQUIRE(s && s->s && out && n_out, tdda::ERR_INVALID, "null argument");
+        const std::string err = s->s->error();
+        TDDA_REQUIRE(err.empty(), tdda::ERR_STATE, "server loop failed: " + err);
+        std::vector<tdda::ServeResult> v;
+        const int k = s->s->poll(v, max);
+        for (int i = 0; i < k; ++i) {
+            out[i].id = v[i].id;
+            out[i].status = v[i].status;
+            out[i].iterations = v[i].iterations;
+            out[i].reliable_updates = v[i].n_ru;
+            out[i].error = v[i].err;
+            out[i].t_submit = v[i].t_submit;
+            out[i].t_admit = v[i].t_admit;
+            out[i].t_done = v[i].t_done;
+            out[i].operator_cached = v[i].op_hit;
+        }
+        *n_out = k;
+    });
+}
+
+int tdda_server_take_field(tdda_server* s, long id, double* E, long n) {
+    return guarded([&] {
+        TDDA_REQUIRE(s && s->s && E, tdda::ERR_INVALID, "null argument");
+        std::vector<double2> v;
+        TDDA_REQUIRE(s->s->take_field(id, v), tdda::ERR_STATE, "no field kept for this request id");
+        TDDA_REQUIRE((long)v.size() == n, tdda::ERR_INVALID, "field buffer must hold 3*nx*ny*nz complex");
+        std::memcpy(E, v.data(), v.size() * sizeof(double2));
+    });
+}
+
+int tdda_server_idle(tdda_server* s, int* idle) {
+    return guarded([&] {
+        TDDA_REQUIRE(s && s->s && idle, tdda::ERR_INVALID, "null argument");
+        const std::string err = s->s->error();
+        TDDA_REQUIRE(err.empty(), tdda::ERR_STATE, "server loop failed: " + err);
+        *idle = s->s->idle() ? 1 : 0;
+    });
+}
+
+int tdda_server_get_stats(tdda_server* s, tdda_server_stats* out) {
+    return guarded([&] {
+        TDDA_REQUIRE(s && s->s && out, tdda::ERR_INVALID, "null argument");
+        tdda::ServerStats st = s->s->stats();
+        out->submitted = st.submitted;
+        out->completed = st.completed;
+        out->chunks = st.chunks;
+        out->rebuilds = st.rebuilds;
+        out->launches = st.launches;
+        out->max_active = st.max_active;
+        out->peak_bytes = (long)st.peak_bytes;
     });
 }
 

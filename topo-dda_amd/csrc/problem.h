@@ -1,0 +1,54 @@
+// A device-resident DDA problem on a full Nx*Ny*Nz box (x-fastest dipoles)
+// with one density value per voxel: the unit of work of the batched solver,
+// the request server and the autograd path.
+#pragma once
+
+#include <complex>
+#include <memory>
+
+#include "runtime.h"
+
+namespace tdda {
+
+struct ProblemDesc {
+    int nx, ny, nz;
+    double d;           // dipole spacing, nm
+    double lam;         // wavelength, nm
+    double nback;       // background refractive index
+    int method;         // 0 FCD, 1 LDR
+    int maxm, maxn;     // image lattice range
+    double lm, ln;      // image periods in cells
+    double nk[3];       // propagation direction
+    double ne0[3];      // polarisation
+    double e0;          // amplitude
+    double mat[4];      // two permittivities (re, im): background, foreground
+};
+
+// Permittivities after AProductCore.cpp:25-33 (divided by nback^2, +0.01 when |m| < 1.01).
+void problem_materials(const ProblemDesc& d, double2* m0, double2* m1);
+OperatorKey problem_operator_key(const ProblemDesc& d, const SiciDevice& tables);
+
+class Problem {
+   public:
+    // op may be null: it is then taken from the operator cache (built if absent).
+    Problem(const ProblemDesc& desc, const SiciDevice& tables, bool refined, hipStream_t st,
+            std::shared_ptr<Operator> op = nullptr, std::unique_ptr<System> reuse = nullptr);
+    // rho: N densities in [0, 1] (clipped), host or device memory; computes
+    // 1/alpha (DDAModel.cpp:317-328, Tools.cpp:353-366) and the incident field
+    // (DDAModel.cpp:312-316) on the device.
+    void set_density(const double* rho, bool on_device, hipStream_t st);
+    // E = al P 3/(eps + 2) (DDAModel.cpp:1185-1192) into out (device, 3N complex)
+    void field(double2* out, hipStream_t st) const;
+    static size_t estimate_bytes(const ProblemDesc& d, bool refined, bool need_operator);
+
+    ProblemDesc desc;
+    double K;
+    double2 m0, m1;
+    int ndip;
+    std::shared_ptr<Operator> op;
+    std::unique_ptr<System> sys;
+    DevBuf<double> rho;
+    bool op_hit = false;
+};
+
+}  // namespace tdda

@@ -12,24 +12,75 @@ namespace tdda {
 // ---------------------------------------------------------------------------
 // memory
 // ---------------------------------------------------------------------------
-static std::atomic<size_t> g_bytes{0};
-size_t device_bytes_in_use() { return g_bytes.load(); }
+// Caching allocator: hipFree synchronises the device, and the server creates
+// and retires systems continuously, so freed blocks go to per-size free lists
+// and are reused.  Callers free a block only after the work using it is done.
+namespace {
+std::mutex g_mem_mu;
+std::map<size_t, std::vector<void*>> g_free;
+size_t g_in_use = 0, g_cached = 0;
+
+size_t round_size(size_t b) {
+    if (b >= (1u << 20)) return (b + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+    return (b + 511) & ~size_t(511);
+}
+
+void release_cached_locked() {
+    for (auto& kv : g_free)
+        for (void* p : kv.second) (void)hipFree(p);
+    g_free.clear();
+    g_cached = 0;
+}
+}  // namespace
+
+size_t device_bytes_in_use() {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    return g_in_use;
+}
+
+size_t device_bytes_cached() {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    return g_cached;
+}
+
+void device_release_cached() {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    release_cached_locked();
+}
 
 void* dev_alloc(size_t bytes) {
+    const size_t sz = round_size(bytes);
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_free.find(sz);
+    if (it != g_free.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        g_cached -= sz;
+        g_in_use += sz;
+        return p;
+    }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, sz);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        throw Error(ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        release_cached_locked();
+        e = hipMalloc(&p, sz);
     }
-    g_bytes += bytes;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(ERR_MEMORY, "hipMalloc of " + std::to_string(sz) + " bytes failed: " + hipGetErrorString(e));
+    }
+    g_in_use += sz;
     return p;
 }
 
 void dev_free(void* p, size_t bytes) {
     if (!p) return;
-    (void)hipFree(p);
-    g_bytes -= bytes;
+    const size_t sz = round_size(bytes);
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_free[sz].push_back(p);
+    g_in_use -= sz;
+    g_cached += sz;
 }
 
 hipStream_t default_stream() { return nullptr; }
@@ -213,6 +264,22 @@ size_t OperatorCache::bytes() const {
     return s;
 }
 
+size_t OperatorCache::trim(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t freed = 0;
+    for (size_t i = 0; i < lru_.size() && freed < bytes;) {
+        auto it = map_.find(lru_[i]);
+        if (it != map_.end() && it->second.use_count() == 1) {
+            freed += it->second->bytes();
+            map_.erase(it);
+            lru_.erase(lru_.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+    return freed;
+}
+
 void OperatorCache::clear() {
     std::lock_guard<std::mutex> lk(mu_);
     map_.clear();
@@ -356,6 +423,14 @@ void System::get_solution(double2* h, hipStream_t st) const { P.download(h, 3ull
 // batches
 // ---------------------------------------------------------------------------
 void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st) {
+    std::vector<KScalars*> ks(systems.size(), nullptr);
+    if (ks_dev)
+        for (size_t j = 0; j < systems.size(); ++j) ks[j] = ks_dev + j;
+    build(systems, ks, refined, st, true);
+}
+
+void Batch::build(const std::vector<System*>& systems, const std::vector<KScalars*>& ks, bool refined,
+                  hipStream_t st, bool interleave_shared) {
     hjobs.clear();
     std::vector<int2> m[7];
     size_t lds[6] = {0, 0, 0, 0, 0, 0};
@@ -363,15 +438,13 @@ void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool re
         System* s = systems[j];
         TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
         GridJob J = s->job;
-        J.ks = ks_dev ? ks_dev + j : nullptr;
+        J.ks = j < ks.size() ? ks[j] : nullptr;
         hjobs.push_back(J);
         const int jj = (int)j;
         for (int t = 0; t < J.tx32; ++t) m[0].push_back(make_int2(jj, t));
         for (int t = 0; t < J.ty32; ++t) m[1].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.tz32; ++t) m[2].push_back(make_int2(jj, t));
         for (int t = 0; t < J.tx64; ++t) m[3].push_back(make_int2(jj, t));
         for (int t = 0; t < J.ty64; ++t) m[4].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.tz64; ++t) m[5].push_back(make_int2(jj, t));
         for (int t = 0; t < J.tupd; ++t) m[6].push_back(make_int2(jj, t));
         lds[0] = std::max(lds[0], lds_bytes_x(J, true));
         lds[1] = std::max(lds[1], lds_bytes_y(J, true));
@@ -380,9 +453,35 @@ void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool re
         lds[4] = std::max(lds[4], lds_bytes_y(J, false));
         lds[5] = std::max(lds[5], lds_bytes_z(J, false));
     }
+    // Z-pass maps: group systems by operator
+    std::vector<std::vector<int>> groups;
+    std::map<const Operator*, size_t> gidx;
+    for (size_t j = 0; j < systems.size(); ++j) {
+        const Operator* o = interleave_shared ? systems[j]->op.get() : nullptr;
+        if (!interleave_shared) { groups.push_back({(int)j}); continue; }
+        auto it = gidx.find(o);
+        if (it == gidx.end()) { gidx[o] = groups.size(); groups.push_back({(int)j}); }
+        else groups[it->second].push_back((int)j);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int2>& mz = pass == 0 ? m[2] : m[5];
+        for (const auto& g : groups) {
+            const int T = pass == 0 ? hjobs[g[0]].tz32 : hjobs[g[0]].tz64;
+            if (g.size() == 1) {
+                for (int t = 0; t < T; ++t) mz.push_back(make_int2(g[0], t));
+                continue;
+            }
+            for (int t0 = 0; t0 < T; t0 += 8) {
+                const int t1 = std::min(T, t0 + 8);
+                for (int j : g)
+                    for (int t = t0; t < t1; ++t) mz.push_back(make_int2(j, t));
+            }
+        }
+    }
     size_t total = 0;
     for (auto& v : m) total += v.size();
-    std::vector<int2> all;
+    std::vector<int2>& all = hmaps_;   // kept alive: the upload is asynchronous
+    all.clear();
     all.reserve(total);
     size_t off[7];
     for (int i = 0; i < 7; ++i) {
@@ -510,16 +609,29 @@ void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st) {
 }
 
 void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5) {
-    TDDA_REQUIRE(!c64 || s.refined, ERR_INVALID, "c64 transforms need a refined system");
-    TDDA_REQUIRE(reps >= 1, ERR_INVALID, "reps must be positive");
-    Batch b;
     std::vector<System*> v{&s};
-    b.build(v, nullptr, s.refined, st);
-    const size_t n3 = 3ull * s.ndip;
-    DevBuf<double2> x(n3), y(n3);
-    x.zero(st);
-    b.hjobs[0].xin = x.get();
-    b.hjobs[0].yout = y.get();
+    time_passes_batch(v, c64, reps, st, ms5);
+}
+
+void time_passes_batch(const std::vector<System*>& v, bool c64, int reps, hipStream_t st, double* ms5) {
+    TDDA_REQUIRE(!v.empty(), ERR_INVALID, "no systems");
+    TDDA_REQUIRE(reps >= 1, ERR_INVALID, "reps must be positive");
+    const bool refined = v[0]->refined;
+    for (System* s : v) {
+        TDDA_REQUIRE(s->refined == refined, ERR_INVALID, "mixed precisions in one batch");
+        TDDA_REQUIRE(!c64 || s->refined, ERR_INVALID, "c64 transforms need a refined system");
+    }
+    Batch b;
+    b.build(v, nullptr, refined, st);
+    std::vector<DevBuf<double2>> xs(v.size()), ys(v.size());
+    for (size_t j = 0; j < v.size(); ++j) {
+        const size_t n3 = 3ull * v[j]->ndip;
+        xs[j].alloc(n3);
+        ys[j].alloc(n3);
+        xs[j].zero(st);
+        b.hjobs[j].xin = xs[j].get();
+        b.hjobs[j].yout = ys[j].get();
+    }
     b.upload_jobs(st);
     for (int k = 0; k < 5; ++k) ms5[k] = 0.0;
     LaunchSet l = b.ls;

@@ -16,7 +16,9 @@ namespace tdda {
 // ---------------------------------------------------------------------------
 // device buffer (move-only RAII)
 // ---------------------------------------------------------------------------
-size_t device_bytes_in_use();
+size_t device_bytes_in_use();     // live blocks (rounded sizes)
+size_t device_bytes_cached();     // freed blocks kept for reuse
+void device_release_cached();     // hipFree every cached block
 
 template <typename T> class DevBuf {
    public:
@@ -130,6 +132,9 @@ class OperatorCache {
     void set_budget(size_t bytes) { budget_ = bytes; }
     size_t bytes() const;
     void clear();
+    // evict least recently used operators nobody holds until `bytes` are freed;
+    // returns the bytes freed
+    size_t trim(size_t bytes);
 
    private:
     mutable std::mutex mu_;
@@ -188,7 +193,13 @@ struct SolveResult {
 // Device-resident batch: descriptors and (job, tile) maps for every pass.
 class Batch {
    public:
+    // ks_dev: scalars of system j at ks_dev + j (or null for plain operator use)
     void build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st);
+    // explicit scalar slot per system; systems that share an Operator get
+    // their Z-pass tiles interleaved so that co-resident blocks read the same
+    // Green's spectrum (blocks congruent mod 8 run on one XCD and share its L2)
+    void build(const std::vector<System*>& systems, const std::vector<KScalars*>& ks, bool refined,
+               hipStream_t st, bool interleave_shared = true);
     void upload_jobs(hipStream_t st);   // after editing hjobs (xin/yout)
     LaunchSet ls;
     std::vector<GridJob> hjobs;
@@ -196,6 +207,7 @@ class Batch {
    private:
     DevBuf<GridJob> djobs_;
     DevBuf<int2> maps_;
+    std::vector<int2> hmaps_;
 };
 
 // Solve every system in the batch with BiCGSTAB (DDAModel.cpp:899-1009).
@@ -209,6 +221,8 @@ void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st);
 // Average ms per launch of the five passes (X fwd, Y fwd, Z, Y inv, X inv)
 // of y = A x + al x over `reps` repetitions, c64 or c128 transforms.
 void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5);
+// same over a batch of systems in one launch per pass
+void time_passes_batch(const std::vector<System*>& v, bool c64, int reps, hipStream_t st, double* ms5);
 
 hipStream_t default_stream();
 

@@ -1,0 +1,274 @@
+// Continuous-batching request server (see serve.h).
+#include "serve.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace tdda {
+
+Server::Server(const ServerOptions& o, const SiciDevice& tables) : opt_(o), tables_(tables) {
+    TDDA_REQUIRE(o.max_slots >= 1 && o.max_slots <= 4096, ERR_INVALID, "max_slots must be in [1, 4096]");
+    TDDA_REQUIRE(o.chunk >= 1, ERR_INVALID, "chunk must be positive");
+    t0_ = std::chrono::steady_clock::now();
+    TDDA_HIP(hipStreamCreate(&main_));
+    TDDA_HIP(hipStreamCreate(&setup_));
+    ks_.alloc(o.max_slots);
+    ks_host_.assign(o.max_slots, KScalars{});
+    slots_.resize(o.max_slots);
+    th_ = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (th_.joinable()) th_.join();
+    (void)hipStreamSynchronize(main_);
+    (void)hipStreamSynchronize(setup_);
+    slots_.clear();
+    pool_.clear();
+    (void)hipStreamDestroy(main_);
+    (void)hipStreamDestroy(setup_);
+}
+
+double Server::now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+void Server::submit(ServeRequest&& r) {
+    const long n = (long)r.desc.nx * r.desc.ny * r.desc.nz;
+    TDDA_REQUIRE(n >= 1 && (long)r.rho.size() == n, ERR_INVALID, "density must hold nx*ny*nz values");
+    r.t_submit = now();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        TDDA_REQUIRE(error_.empty(), ERR_STATE, "server stopped after an error: " + error_);
+        pending_.push_back(std::move(r));
+        stats_.submitted++;
+    }
+    cv_.notify_all();
+}
+
+int Server::poll(std::vector<ServeResult>& out, int max) {
+    std::lock_guard<std::mutex> lk(mu_);
+    int k = 0;
+    while (!done_.empty() && k < max) {
+        out.push_back(done_.front());
+        done_.pop_front();
+        ++k;
+    }
+    return k;
+}
+
+bool Server::take_field(long id, std::vector<double2>& E) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = fields_.find(id);
+    if (it == fields_.end()) return false;
+    E = std::move(it->second);
+    fields_.erase(it);
+    return true;
+}
+
+bool Server::idle() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return pending_.empty() && busy_ == 0;
+}
+
+ServerStats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return stats_;
+}
+
+std::string Server::error() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return error_;
+}
+
+// Admit queued requests in arrival order while a slot is free and the memory
+// estimate fits; their operators and densities are set up on the setup stream.
+std::vector<int> Server::admit() {
+    std::vector<int> joined;
+    while (true) {
+        ServeRequest req;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (pending_.empty()) break;
+            int free_slot = -1;
+            for (int s = 0; s < (int)slots_.size(); ++s)
+                if (!slots_[s].used) { free_slot = s; break; }
+            if (free_slot < 0) break;
+            const ProblemDesc& d = pending_.front().desc;
+            const size_t need = Problem::estimate_bytes(d, opt_.refined, true);
+            if (busy_ > 0 && device_bytes_in_use() + need > opt_.hbm_budget) {
+                // drop cached operators nobody uses, then re-check
+                OperatorCache::instance().trim(device_bytes_in_use() + need - opt_.hbm_budget);
+                if (device_bytes_in_use() + need > opt_.hbm_budget) break;   // wait for memory
+            }
+            req = std::move(pending_.front());
+            pending_.pop_front();
+            busy_++;
+            slots_[free_slot].used = true;
+            joined.push_back(free_slot);
+        }
+        Slot& sl = slots_[joined.back()];
+        const auto key = std::make_tuple(req.desc.nx, req.desc.ny, req.desc.nz);
+        std::unique_ptr<System> reuse;
+        auto pit = pool_.find(key);
+        if (pit != pool_.end() && !pit->second.empty()) {
+            reuse = std::move(pit->second.back());
+            pit->second.pop_back();
+        }
+        sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
+        sl.prob->set_density(req.rho.data(), false, setup_);
+        sl.prob->sys->P.zero(setup_);
+        sl.op_hit = sl.prob->op_hit ? 1 : 0;
+        sl.t_admit = now();
+        sl.req = std::move(req);
+        sl.req.rho.clear();
+        sl.req.rho.shrink_to_fit();
+    }
+    return joined;
+}
+
+void Server::retire(const std::vector<int>& fin) {
+    if (fin.empty()) return;
+    std::vector<DevBuf<double2>> ebuf(fin.size());
+    std::vector<std::vector<double2>> eh(fin.size());
+    if (opt_.keep_fields) {
+        for (size_t k = 0; k < fin.size(); ++k) {
+            Problem& p = *slots_[fin[k]].prob;
+            ebuf[k].alloc(3ull * p.ndip);
+            p.field(ebuf[k].get(), main_);
+            eh[k].resize(3ull * p.ndip);
+            ebuf[k].download(eh[k].data(), eh[k].size(), main_);
+        }
+        TDDA_HIP(hipStreamSynchronize(main_));
+    }
+    const double t = now();
+    std::lock_guard<std::mutex> lk(mu_);
+    for (size_t k = 0; k < fin.size(); ++k) {
+        Slot& sl = slots_[fin[k]];
+        const KScalars& ks = ks_host_[fin[k]];
+        ServeResult r;
+        r.id = sl.req.id;
+        r.status = ks.status;
+        r.iterations = ks.iterations;
+        r.n_ru = ks.n_ru;
+        r.err = ks.err;
+        r.t_submit = sl.req.t_submit;
+        r.t_admit = sl.t_admit;
+        r.t_done = t;
+        r.op_hit = sl.op_hit;
+        done_.push_back(r);
+        if (opt_.keep_fields) fields_[r.id] = std::move(eh[k]);
+        const auto key = std::make_tuple(sl.req.desc.nx, sl.req.desc.ny, sl.req.desc.nz);
+        sl.prob->sys->op.reset();   // let the operator cache evict it
+        pool_[key].push_back(std::move(sl.prob->sys));
+        sl.prob.reset();
+        sl.used = false;
+        busy_--;
+        stats_.completed++;
+    }
+}
+
+void Server::rebuild() {
+    std::vector<System*> sys;
+    std::vector<KScalars*> ks;
+    for (int s : active_) {
+        sys.push_back(slots_[s].prob->sys.get());
+        ks.push_back(ks_.get() + s);
+    }
+    if (!sys.empty()) batch_.build(sys, ks, opt_.refined, main_);
+    std::lock_guard<std::mutex> lk(mu_);
+    stats_.rebuilds++;
+    stats_.max_active = std::max(stats_.max_active, (int)active_.size());
+}
+
+void Server::loop() {
+    try {
+        TwiddlePools tw = PlanRegistry::instance().pools();
+        hipEvent_t setup_done;
+        TDDA_HIP(hipEventCreateWithFlags(&setup_done, hipEventDisableTiming));
+        while (true) {
+            bool ran = false;
+            if (!active_.empty()) {
+                tw = PlanRegistry::instance().pools();
+                for (int k = 0; k < opt_.chunk; ++k) {
+                    launch_matvec(batch_.ls, tw, opt_.refined, XF_P, XI_V, GATE_ACTIVE, main_);
+                    launch_matvec(batch_.ls, tw, opt_.refined, XF_T, XI_S, GATE_ACTIVE, main_);
+                    launch_update(batch_.ls, GATE_ACTIVE, main_);
+                    if (opt_.refined) launch_matvec(batch_.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, main_);
+                }
+                TDDA_HIP(hipMemcpyAsync(ks_host_.data(), ks_.get(), ks_host_.size() * sizeof(KScalars),
+                                        hipMemcpyDeviceToHost, main_));
+                ran = true;
+            }
+            // set up arrivals while the batch runs
+            std::vector<int> joined = admit();
+            if (!joined.empty()) TDDA_HIP(hipEventRecord(setup_done, setup_));
+            if (ran) TDDA_HIP(hipStreamSynchronize(main_));
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (ran) {
+                    stats_.chunks++;
+                    stats_.launches += (long)opt_.chunk * (opt_.refined ? 16 : 11);
+                }
+                stats_.peak_bytes = std::max(stats_.peak_bytes, device_bytes_in_use());
+            }
+            bool changed = false;
+            if (ran) {
+                std::vector<int> fin, keep;
+                for (int s : active_) (ks_host_[s].status != JOB_ACTIVE ? fin : keep).push_back(s);
+                if (!fin.empty()) {
+                    retire(fin);
+                    active_ = keep;
+                    changed = true;
+                }
+            }
+            if (!joined.empty()) {
+                TDDA_HIP(hipStreamWaitEvent(main_, setup_done, 0));
+                std::vector<KScalars> init(joined.size());
+                std::vector<System*> js;
+                std::vector<KScalars*> jk;
+                for (size_t k = 0; k < joined.size(); ++k) {
+                    std::memset(&init[k], 0, sizeof(KScalars));
+                    init[k].tol = opt_.tol;
+                    init[k].maxit = opt_.maxit;
+                    init[k].refined = opt_.refined ? 1 : 0;
+                    init[k].ru_delta = opt_.ru_delta;
+                    init[k].status = JOB_ACTIVE;
+                    TDDA_HIP(hipMemcpyAsync(ks_.get() + joined[k], &init[k], sizeof(KScalars),
+                                            hipMemcpyHostToDevice, main_));
+                    js.push_back(slots_[joined[k]].prob->sys.get());
+                    jk.push_back(ks_.get() + joined[k]);
+                }
+                Batch ib;
+                ib.build(js, jk, opt_.refined, main_, false);
+                launch_init(ib.ls, false, main_);
+                TDDA_HIP(hipStreamSynchronize(main_));   // ib's buffers die here
+                for (int s : joined) active_.push_back(s);
+                changed = true;
+            }
+            if (changed) rebuild();
+            if (active_.empty()) {
+                std::unique_lock<std::mutex> lk(mu_);
+                if (stop_) break;
+                if (pending_.empty())
+                    cv_.wait_for(lk, std::chrono::milliseconds(50), [this] { return stop_ || !pending_.empty(); });
+                if (stop_) break;
+            } else {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (stop_) break;
+            }
+        }
+        (void)hipEventDestroy(setup_done);
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        error_ = e.what();
+        busy_ = 0;
+        pending_.clear();
+    }
+}
+
+}  // namespace tdda

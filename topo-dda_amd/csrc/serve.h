@@ -1,0 +1,107 @@
+// Request server: continuous batching of DDA solves of uneven shape on one GPU.
+//
+// Requests (box shape, wavelength, incidence, density) are queued by submit().
+// A loop thread admits them in arrival order while a slot is free and the
+// device-memory estimate fits the budget, builds (or reuses from the cache)
+// their Green's spectra on a setup stream while the compute stream runs the
+// current batch, and advances every admitted solve together: each launch of
+// the five FFT passes and the Krylov kernels covers all active requests.
+// Finished requests leave the batch after every chunk of iterations; their
+// slots and device buffers are reused by the next arrivals.
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <tuple>
+
+#include "problem.h"
+
+namespace tdda {
+
+struct ServerOptions {
+    bool refined = true;             // c64 Krylov + FP64 reliable updates; false: FP64
+    int maxit = 100000;
+    double tol = 1e-5;
+    double ru_delta = 0.1;
+    size_t hbm_budget = size_t(200) << 30;
+    int max_slots = 64;
+    int chunk = 4;                   // iterations between host checks
+    bool keep_fields = true;         // keep E for take_field()
+};
+
+struct ServeRequest {
+    long id = 0;
+    ProblemDesc desc{};
+    std::vector<double> rho;
+    double t_submit = 0.0;
+};
+
+struct ServeResult {
+    long id = 0;
+    int status = JOB_IDLE;
+    int iterations = 0;
+    int n_ru = 0;
+    double err = 0.0;
+    double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;   // seconds since server start
+    int op_hit = 0;
+};
+
+struct ServerStats {
+    long submitted = 0, completed = 0, chunks = 0, rebuilds = 0, launches = 0;
+    int max_active = 0;
+    size_t peak_bytes = 0;
+};
+
+class Server {
+   public:
+    Server(const ServerOptions& o, const SiciDevice& tables);
+    ~Server();
+    void submit(ServeRequest&& r);
+    int poll(std::vector<ServeResult>& out, int max);
+    bool take_field(long id, std::vector<double2>& E);
+    bool idle();
+    double now() const;
+    ServerStats stats();
+    std::string error();   // first loop failure, if any
+
+   private:
+    struct Slot {
+        bool used = false;
+        std::unique_ptr<Problem> prob;
+        ServeRequest req;
+        double t_admit = 0.0;
+        int op_hit = 0;
+    };
+    void loop();
+    std::vector<int> admit();
+    void retire(const std::vector<int>& slots);
+    void rebuild();
+
+    ServerOptions opt_;
+    SiciDevice tables_;
+    std::chrono::steady_clock::time_point t0_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<ServeRequest> pending_;
+    std::deque<ServeResult> done_;
+    std::map<long, std::vector<double2>> fields_;
+    bool stop_ = false;
+    std::string error_;
+    ServerStats stats_;
+
+    // loop-thread state
+    hipStream_t main_ = nullptr, setup_ = nullptr;
+    DevBuf<KScalars> ks_;
+    std::vector<KScalars> ks_host_;
+    std::vector<Slot> slots_;
+    std::vector<int> active_;
+    std::map<std::tuple<int, int, int>, std::vector<std::unique_ptr<System>>> pool_;
+    Batch batch_;
+    int busy_ = 0;   // requests admitted but not finished (guarded by mu_)
+};
+
+}  // namespace tdda

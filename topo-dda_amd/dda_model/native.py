@@ -1,0 +1,236 @@
+"""ctypes binding of the batched-problem and server entry points of
+include/topodda.h (the parts with no reference counterpart).
+
+Everything here calls the in-tree libtopodda.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_LIB = None
+PREC = {"fp64": 0, "fp32_refined": 1}
+STATUS = {0: "active", 1: "converged", 2: "breakdown", 3: "maxit", 4: "idle"}
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("d", ctypes.c_double), ("lam", ctypes.c_double), ("nback", ctypes.c_double),
+                ("method", ctypes.c_int), ("maxm", ctypes.c_int), ("maxn", ctypes.c_int),
+                ("lm", ctypes.c_double), ("ln", ctypes.c_double),
+                ("n_k", ctypes.c_double * 3), ("n_e0", ctypes.c_double * 3), ("e0", ctypes.c_double),
+                ("material", ctypes.c_double * 4)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("error", ctypes.c_double),
+                ("reliable_updates", ctypes.c_int), ("ms", ctypes.c_double)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("maxit", ctypes.c_int), ("tol", ctypes.c_double),
+                ("ru_delta", ctypes.c_double), ("hbm_budget_gb", ctypes.c_double), ("max_slots", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("keep_fields", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_long), ("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("reliable_updates", ctypes.c_int), ("error", ctypes.c_double), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("operator_cached", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_long), ("completed", ctypes.c_long), ("chunks", ctypes.c_long),
+                ("rebuilds", ctypes.c_long), ("launches", ctypes.c_long), ("max_active", ctypes.c_int),
+                ("peak_bytes", ctypes.c_long)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtopodda.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+        L.tdda_last_error.restype = ctypes.c_char_p
+        L.tdda_problem_create.argtypes = [ctypes.POINTER(ProblemDesc), ctypes.c_int, dp, dp, ctypes.c_long,
+                                          ctypes.c_double, ctypes.POINTER(vp)]
+        L.tdda_problem_destroy.argtypes = [vp]
+        L.tdda_problem_set_density.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.tdda_problems_solve.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.POINTER(SolveInfo)]
+        L.tdda_problem_field.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.tdda_problem_polarization.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.tdda_problems_time_passes.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_server_create.argtypes = [ctypes.POINTER(ServerOpts), dp, dp, ctypes.c_long, ctypes.c_double,
+                                         ctypes.POINTER(vp)]
+        L.tdda_server_destroy.argtypes = [vp]
+        L.tdda_server_submit.argtypes = [vp, ctypes.c_long, ctypes.POINTER(ProblemDesc), dp, ctypes.c_long]
+        L.tdda_server_poll.argtypes = [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.tdda_server_take_field.argtypes = [vp, ctypes.c_long, dp, ctypes.c_long]
+        L.tdda_server_idle.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.tdda_device_bytes.restype = ctypes.c_long
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().tdda_last_error().decode()
+        if rc == 2:
+            raise ValueError(msg)
+        raise RuntimeError(msg or f"topodda error {rc}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Request:
+    """One DDA solve: a full box with a density per voxel (x fastest)."""
+    shape: tuple
+    d: float
+    lam: float
+    rho: np.ndarray
+    n_k: tuple = (0.0, 0.0, 1.0)
+    n_e0: tuple = (1.0, 0.0, 0.0)
+    e0: float = 1.0
+    material: tuple = (1.0 + 0j, 5.96 + 3.89j)
+    nback: float = 1.0
+    method: str = "LDR"
+    images: int = 0
+    period: tuple = (0.0, 0.0)
+
+    def desc(self):
+        nx, ny, nz = self.shape
+        m = [complex(self.material[0]), complex(self.material[1])]
+        return ProblemDesc(nx, ny, nz, self.d, self.lam, self.nback, 0 if self.method == "FCD" else 1,
+                           self.images, self.images, self.period[0], self.period[1],
+                           (ctypes.c_double * 3)(*self.n_k), (ctypes.c_double * 3)(*self.n_e0), self.e0,
+                           (ctypes.c_double * 4)(m[0].real, m[0].imag, m[1].real, m[1].imag))
+
+
+def _tables(tables):
+    if tables is None:
+        return None, None, 0, 0.1
+    si, ci, delta = tables
+    si = np.ascontiguousarray(si, dtype=np.float64)
+    ci = np.ascontiguousarray(ci, dtype=np.float64)
+    return si, ci, si.size, delta
+
+
+class Problem:
+    """A device-resident problem (operator from the device cache)."""
+
+    def __init__(self, req: Request, precision="fp32_refined", tables=None):
+        si, ci, n, delta = _tables(tables)
+        self._keep = (si, ci)
+        h = ctypes.c_void_p()
+        check(lib().tdda_problem_create(ctypes.byref(req.desc()), PREC[precision],
+                                        _dp(si) if si is not None else None,
+                                        _dp(ci) if ci is not None else None, n, delta, ctypes.byref(h)))
+        self.h = h
+        self.n = int(np.prod(req.shape))
+        self.set_density(req.rho)
+
+    def set_density(self, rho):
+        """rho: numpy array (host) or a torch CUDA tensor of float64 (device)."""
+        if hasattr(rho, "data_ptr"):
+            check(lib().tdda_problem_set_density(self.h, ctypes.c_void_p(rho.data_ptr()), 1))
+        else:
+            r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+            assert r.size == self.n
+            check(lib().tdda_problem_set_density(self.h, r.ctypes.data_as(ctypes.c_void_p), 0))
+
+    def field(self):
+        out = np.empty(3 * self.n, dtype=np.complex128)
+        check(lib().tdda_problem_field(self.h, out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def polarization(self):
+        out = np.empty(3 * self.n, dtype=np.complex128)
+        check(lib().tdda_problem_polarization(self.h, out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.tdda_problem_destroy(self.h)
+            self.h = None
+
+
+def solve(problems, maxit=100000, tol=1e-5):
+    """Solve several problems in one batch; returns a list of info dicts."""
+    arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
+    infos = (SolveInfo * len(problems))()
+    check(lib().tdda_problems_solve(arr, len(problems), maxit, tol, infos))
+    return [dict(status=STATUS[i.status], iterations=i.iterations, error=i.error,
+                 reliable_updates=i.reliable_updates, ms=i.ms) for i in infos]
+
+
+def time_passes(problems, c64=True, reps=20):
+    """Average ms per batched launch of (X fwd, Y fwd, Z, Y inv, X inv)."""
+    arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
+    ms = np.zeros(5)
+    check(lib().tdda_problems_time_passes(arr, len(problems), int(c64), reps, _dp(ms)))
+    return ms
+
+
+class Server:
+    """Continuous-batching request server (C++ loop thread)."""
+
+    def __init__(self, precision="fp32_refined", maxit=100000, tol=1e-5, ru_delta=0.1, hbm_budget_gb=200.0,
+                 max_slots=64, chunk=4, keep_fields=True, tables=None):
+        si, ci, n, delta = _tables(tables)
+        self._keep = (si, ci)
+        o = ServerOpts(PREC[precision], maxit, tol, ru_delta, hbm_budget_gb, max_slots, chunk, int(keep_fields))
+        h = ctypes.c_void_p()
+        check(lib().tdda_server_create(ctypes.byref(o), _dp(si) if si is not None else None,
+                                       _dp(ci) if ci is not None else None, n, delta, ctypes.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def submit(self, rid, req: Request):
+        rho = np.ascontiguousarray(req.rho, dtype=np.float64).reshape(-1)
+        self.sizes[rid] = rho.size
+        check(lib().tdda_server_submit(self.h, rid, ctypes.byref(req.desc()), _dp(rho), rho.size))
+
+    def poll(self, max_results=256):
+        buf = (Result * max_results)()
+        n = ctypes.c_int(0)
+        check(lib().tdda_server_poll(self.h, buf, max_results, ctypes.byref(n)))
+        return [dict(id=r.id, status=STATUS[r.status], iterations=r.iterations,
+                     reliable_updates=r.reliable_updates, error=r.error, t_submit=r.t_submit,
+                     t_admit=r.t_admit, t_done=r.t_done, operator_cached=bool(r.operator_cached))
+                for r in buf[:n.value]]
+
+    def field(self, rid):
+        out = np.empty(3 * self.sizes[rid], dtype=np.complex128)
+        check(lib().tdda_server_take_field(self.h, rid, _dp(out.view(np.float64)), out.size))
+        return out
+
+    def idle(self):
+        v = ctypes.c_int(0)
+        check(lib().tdda_server_idle(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().tdda_server_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().tdda_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
