@@ -120,6 +120,28 @@ def cpu_baseline(reqs, budget_s=20.0):
     return done, setup_s, solve_s, iters
 
 
+PROBE_PER_SHAPE = 8
+
+
+def probe_shapes():
+    return [s for _ in range(PROBE_PER_SHAPE) for s in SHAPES]
+
+
+def roofline_probe(native, reps):
+    """One batched launch per pass over a serving-like batch (8 problems of
+    every shape, distinct wavelengths so no two share a Green's spectrum, c64),
+    and a device copy of known size for calibrating byte counters."""
+    import torch
+    probs = [native.Problem(native.Request(shape=s, d=15.0, lam=500.0 + 3.0 * i, rho=np.full(int(np.prod(s)), 0.5)),
+                            precision="fp32_refined") for i, s in enumerate(probe_shapes())]
+    ms5 = native.time_passes(probs, c64=True, reps=reps)
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")   # 256 MiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return ms5, probs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,12 +153,19 @@ def main():
     ap.add_argument("--chunk", type=int, default=4)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the batched pass timing and a 256 MiB copy (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (no multi-GPU path is built)",
               file=sys.stderr)
         sys.exit(2)
     from dda_model import native
+
+    if a.roofline_only:
+        ms5, _ = roofline_probe(native, 30)
+        print(json.dumps({"pass_ms": [float(x) for x in ms5], "copy_bytes": 2 * (256 << 20)}), flush=True)
+        return
 
     reqs = make_requests(native, a.requests)
     print(f"# {len(reqs)} requests generated", file=sys.stderr, flush=True)
@@ -174,15 +203,13 @@ def main():
     print(f"# fp64 single-solve: {fp64_rate:.2f} solves/s", file=sys.stderr, flush=True)
 
     # roofline: one batched launch per pass over one problem of every shape
-    probs = [native.Problem(native.Request(shape=s, d=15.0, lam=600.0, rho=np.full(int(np.prod(s)), 0.5)),
-                            precision="fp32_refined") for s in SHAPES]
-    ms5 = native.time_passes(probs, c64=True, reps=30)
-    Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in SHAPES]
-    ns = [3 * int(np.prod(s)) for s in SHAPES]
+    ms5, probs = roofline_probe(native, 30)
+    Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in probe_shapes()]
+    ns = [3 * int(np.prod(s)) for s in probe_shapes()]
     b_z = sum(8 * (4 * M + 6 * M) for M in Ms)
     b_mv = sum(8 * (2 * n + 12 * M) + 8 * 6 * M for n, M in zip(ns, Ms))
     roof = {"bound": "hbm", "kernel": "k_zpass<float>: z FFT + Green multiply + inverse z FFT, one launch "
-                                      "over a 5-request batch (one per shape)",
+                                      f"over a {len(Ms)}-request batch ({PROBE_PER_SHAPE} per shape)",
             "achieved": round(b_z / (ms5[2] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b_z / (ms5[2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
             "algorithmic_bytes": b_z, "z_ms": round(float(ms5[2]), 5),

@@ -138,6 +138,9 @@ int tdda_model_time_passes(tdda_model* m, int reps, double* ms5);
 
 /* Bytes of device memory currently held by the library. */
 long tdda_device_bytes(void);
+/* Drop cached Green's spectra nobody uses and return cached device blocks to
+ * HIP (no reference counterpart: the reference frees on model destruction). */
+int tdda_trim_caches(void);
 
 /* ------------------------------------------------------------------------
  * Device-resident problems, batched solves and the request server.
@@ -174,6 +177,17 @@ int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_so
 int tdda_problem_field(tdda_problem* p, double* E, int on_device);
 /* polarisation P: 3N complex, to host or device memory */
 int tdda_problem_polarization(tdda_problem* p, double* P, int on_device);
+#define TDDA_OBJ_INTEGRATED 0 /* log sum_i sum_c |al_i P_ic|^power (ObjDDAModel.cpp:130-212) */
+#define TDDA_OBJ_TARGETS 1    /* mean |E|^2 over external points (DDAModel::solve_E, DDAModel.cpp:1153-1183) */
+/* Objective of the current solution.  targets: ntargets (x,y,z) in nm for
+ * TDDA_OBJ_TARGETS; target_fields (nullable): 3*ntargets complex E. */
+int tdda_problem_objective(tdda_problem* p, int kind, int power, const double* targets, int ntargets, double* value,
+                           double* target_fields);
+/* d objective / d rho for every voxel by one adjoint solve (DDAModel::calculateGradients,
+ * src/DDAModel.cpp:398-469, with closed-form partials); grad: nx*ny*nz doubles,
+ * host or device.  value (nullable): the objective; adj (nullable): adjoint solve record. */
+int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* targets, int ntargets, int maxit,
+                          double tol, double* grad, int on_device, double* value, tdda_solve_info* adj);
 /* Average ms per launch of the five matvec passes over all n problems in one
  * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
  * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */

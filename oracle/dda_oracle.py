@@ -556,6 +556,104 @@ class OracleDDAModel:
         return grad
 
 
+class OracleFullBox:
+    """A full Nx*Ny*Nz box (x-fastest dipoles) with one density per voxel: the
+    same operator, polarizability, incident field and BiCGSTAB as the model
+    above, without the parameter classes (eps from rho per voxel,
+    DDAModel.cpp:319-328).  Densities are clipped to [0, 1].  Objectives:
+    IntegratedE over all dipoles (ObjDDAModel.cpp:130-212) and the mean |E|^2
+    over external points, E(r) = E_inc(r) - sum_j A(r - r_j) P_j with the
+    undamped single-image interaction (DDAModel::solve_E, DDAModel.cpp:1153-1183,
+    incident field E0 e exp(i K k.r) in every component)."""
+
+    def __init__(self, shape, d, lam, rho, n_k=(0, 0, 1), n_e0=(1, 0, 0), e0=1.0,
+                 material=(1.0 + 0j, 5.96 + 3.89j), nback=1.0, method="LDR", images=0, period=(0, 0),
+                 tables=None, workers=1):
+        self.shape = tuple(shape)
+        nx, ny, nz = shape
+        self.N = nx * ny * nz
+        self.d, self.lam = float(d), float(lam)
+        self.nK, self.nE0, self.E0 = np.asarray(n_k, float), np.asarray(n_e0, float), float(e0)
+        self.K = wavenumber(self.lam, nback)
+        self.material = adjust_material(material, nback)
+        self.method = METHOD_FCD if method == "FCD" else METHOD_LDR
+        si, ci, delta = tables if tables is not None else (np.zeros(2), np.zeros(2), 0.1)
+        self.tables = (si, ci, delta)
+        self.geometry = generate_geometry(nx, ny, nz)
+        self.cent = green_offsets(nx, ny, nz, self.d, self.K, self.method, images, images,
+                                  period[0] * self.d, period[1] * self.d, si, ci, delta, octant_only=True)
+        self.op = FFTOperator(self.cent, self.geometry, nx, ny, nz, workers=workers)
+        self.Einc = incident_field(self.geometry, self.K, self.d, self.E0, self.nE0, self.nK)
+        self.set_density(rho)
+
+    def set_density(self, rho):
+        self.rho = np.clip(np.asarray(rho, float).reshape(-1), 0.0, 1.0)
+        self.eps = interp_eps(self.rho, self.material)
+        self.al = np.repeat(1.0 / get_alpha(self.K, self.d, self.eps, self.nE0, self.nK), 3)
+
+    def apply(self, b):
+        return self.op.apply(b) + self.al * b
+
+    def solve(self, tol=1e-12, maxit=100000, rhs=None):
+        rhs = self.Einc if rhs is None else rhs
+        P, info = bicgstab_reference(self.apply, rhs, np.zeros(3 * self.N, complex), maxit, tol)
+        if rhs is self.Einc:
+            self.P, self.info = P, info
+        return P, info
+
+    def field(self):
+        return self.al * self.P / ((2.0 + np.repeat(self.eps, 3)) / 3.0)
+
+    def objective_integrated(self, power=2):
+        return math.log(float(np.sum(np.abs(self.al * self.P) ** power)))
+
+    def _pair(self, r):
+        si, ci, delta = self.tables
+        return green_point(self.K, self.d, self.method, *r, si=si, ci=ci, delta=delta)
+
+    def target_fields(self, targets):
+        g = self.geometry.reshape(-1, 3) * self.d
+        out = []
+        for t in np.asarray(targets, float).reshape(-1, 3):
+            ph = self.K * float(self.nK @ t)
+            e = self.E0 * self.nE0 * complex(math.cos(ph), math.sin(ph))
+            s = np.zeros(3, complex)
+            for j in range(self.N):
+                s += self._pair(t - g[j]) @ self.P[3 * j:3 * j + 3]
+            out.append(e - s)
+        return np.array(out)
+
+    def objective_targets(self, targets):
+        E = self.target_fields(targets)
+        return float(np.sum(np.abs(E) ** 2)) / E.shape[0]
+
+    def gradient(self, kind="integrated", power=2, targets=None, tol=1e-12):
+        """d objective / d rho per voxel: closed-form partials, one adjoint solve."""
+        w = self.al * self.P
+        dal = np.repeat(dal_deps(self.K, self.d, self.eps, self.nE0, self.nK) * (self.material[1] - self.material[0]), 3)
+        explicit = np.zeros(3 * self.N)
+        if kind == "integrated":
+            eint = float(np.sum(np.abs(w) ** power))
+            aw = np.abs(w)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                s = np.where(aw > 0, power * aw ** (power - 2), 0.0) if power != 2 else np.full_like(aw, 2.0)
+            devp = s * np.conj(w) * self.al / eint
+            explicit = s * np.real(np.conj(w) * dal * self.P) / eint
+        else:
+            E = self.target_fields(targets)
+            T = E.shape[0]
+            g = self.geometry.reshape(-1, 3) * self.d
+            devp = np.zeros(3 * self.N, complex)
+            for j in range(self.N):
+                acc = np.zeros(3, complex)
+                for t, tt in enumerate(np.asarray(targets, float).reshape(-1, 3)):
+                    acc += self._pair(tt - g[j]) @ np.conj(E[t])
+                devp[3 * j:3 * j + 3] = -2.0 / T * acc
+        lam, _ = self.solve(tol=tol, rhs=devp)
+        contrib = explicit - np.real(lam * dal * self.P)
+        return contrib.reshape(-1, 3).sum(axis=1)
+
+
 def reference_gradient_sequential(model, eps, origin):
     """Literal restatement of devx_and_Adevxp and devp with the reference's
     incremental E_int bookkeeping (DDAModel.cpp:471-578, ObjDDAModel.cpp:130-157),

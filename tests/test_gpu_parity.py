@@ -83,15 +83,23 @@ def test_operator_apply_scattered_geometry(dm, O, tables):
     shape = (6, 5, 4)
     full = O.generate_geometry(*shape).reshape(-1, 3)
     rng = np.random.default_rng(11)
-    pick = np.sort(rng.permutation(len(full))[:70])
-    pick = rng.permutation(pick)
+    # whole z-columns of a random subset of (x, y), in shuffled order: the
+    # reference's parameterisation needs NFpara = N / Nz columns
+    cols = rng.permutation(30)[:17]
+    keep = np.isin(full[:, 0] + 6 * full[:, 1], cols)
+    pick = rng.permutation(np.where(keep)[0])
     geo = full[pick].reshape(-1)
-    diel = np.repeat(rng.uniform(0, 1, 70), 3)
+    n = pick.size
+    diel = np.repeat(rng.uniform(0, 1, n), 3)
     a = _args(O, tables, shape, geometry=geo, diel=diel)
     g = dm.DDAModel(*a)
     o = O.OracleDDAModel(*a)
-    x = rng.standard_normal(210) + 1j * rng.standard_normal(210)
+    x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
     assert _rel(g.applyOperator(x), o.apply(x)) < 1e-11
+    # and a solve through the box map
+    g.solveElectricField(np.zeros(3 * n, complex), 10000, 1e-11)
+    o.solveElectricField(np.zeros(3 * n, complex), 10000, 1e-11)
+    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-8
 
 
 def test_golden_wrapper_objective(dm, golden_inputs):
@@ -119,8 +127,11 @@ def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
     info = g.solveInfo(0)
     _log(f"[gpu] iterations gpu {info['iterations']} oracle {o.info.iterations}")
     assert abs(info["iterations"] - o.info.iterations) <= 1
-    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
-    assert g.calculateObjective() == pytest.approx(o.calculateObjective(), rel=1e-8)
+    # two solves that each stop at a 1e-5 residual differ by about the error of
+    # either (the rounding of the FFTs steers the Krylov trajectory); the
+    # tight comparison below is the parity statement
+    assert _rel(g.getElectricField(), o.getElectricField()) < 5e-5
+    assert g.calculateObjective() == pytest.approx(o.calculateObjective(), rel=1e-5)
     # tightly converged: both solve the same system
     g.solveElectricField(p0, 100000, 1e-11)
     o.solveElectricField(p0, 100000, 1e-11)

@@ -369,3 +369,46 @@ def test_reference_gradient_close_to_exact(tables):
         errs.append(np.linalg.norm(gr - ge) / np.linalg.norm(ge))
     assert errs[0] < 0.1
     assert 5 < errs[0] / errs[1] < 20   # first-order convergence in eps
+
+
+# ---------------------------------------------------------------------------
+# full-box (per-voxel density) oracle used for the device problems
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["integrated", "targets"])
+def test_full_box_gradient_matches_central_fd(kind):
+    rng = np.random.default_rng(8)
+    shape = (4, 3, 2)
+    rho = rng.uniform(0.1, 0.9, 24)
+    targets = [(15.0, 10.0, 60.0), (30.0, 20.0, 75.0)]
+    box = O.OracleFullBox(shape, 12.0, 542.0, rho, n_k=(0.2, 0.0, math.sqrt(0.96)), n_e0=(0.0, 1.0, 0.0))
+    box.solve(tol=1e-13)
+    f = (lambda: box.objective_integrated(2)) if kind == "integrated" else (lambda: box.objective_targets(targets))
+    g = box.gradient(kind=kind, targets=targets, tol=1e-13)
+    h = 1e-5
+    for i in (0, 7, 23):
+        vals = []
+        for s in (1, -1):
+            r = rho.copy()
+            r[i] += s * h
+            box.set_density(r)
+            box.solve(tol=1e-13)
+            vals.append(f())
+        box.set_density(rho)
+        box.solve(tol=1e-13)
+        assert g[i] == pytest.approx((vals[0] - vals[1]) / (2 * h), rel=1e-5, abs=1e-10)
+
+
+def test_full_box_matches_model_when_layers_agree(tables):
+    """With z-uniform densities the per-voxel box and the z-extruded model are
+    the same system."""
+    rng = np.random.default_rng(9)
+    shape = (4, 4, 3)
+    col = rng.uniform(0, 1, 16)
+    rho = np.tile(col, 3)
+    box = O.OracleFullBox(shape, 15.0, 542.0, rho, method="FCD", images=2, period=(4, 4), tables=tables)
+    box.solve(tol=1e-12)
+    a = O.wrapper_args(list(shape), 15.0, np.repeat(rho, 3), [1.0 + 0j, 5.96 + 3.89j], [0, 0, 1], [1, 0, 0], 542.0,
+                       symmetry_method="None", apc_max_m=2, apc_max_n=2, apc_method_name="FCD", tables=tables)
+    m = O.OracleDDAModel(*a)
+    m.solveElectricField(np.zeros(3 * m.N, complex), 10000, 1e-12)
+    assert np.allclose(box.field(), m.getElectricField(), rtol=1e-9, atol=1e-12)

@@ -4,6 +4,7 @@
 
 #include "../../include/topodda.h"
 #include "model.h"
+#include "objective.h"
 #include "serve.h"
 
 namespace tdda {
@@ -43,6 +44,14 @@ extern "C" {
 const char* tdda_last_error(void) { return tdda::get_last_error(); }
 
 long tdda_device_bytes(void) { return (long)tdda::device_bytes_in_use(); }
+
+int tdda_trim_caches(void) {
+    return guarded([&] {
+        tdda::OperatorCache::instance().trim(~size_t(0));
+        TDDA_HIP(hipDeviceSynchronize());
+        tdda::device_release_cached();
+    });
+}
 
 int tdda_model_create(const tdda_model_args* a, tdda_model** out) {
     return guarded([&] {
@@ -359,6 +368,44 @@ int tdda_problem_polarization(tdda_problem* p, double* P, int on_device) {
         const size_t n3 = 3ull * p->p->ndip;
         TDDA_HIP(hipMemcpy(P, p->p->sys->P.get(), n3 * sizeof(double2),
                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    });
+}
+
+static tdda::ObjectiveSpec obj_spec(int kind, int power, const double* targets, int nt) {
+    tdda::ObjectiveSpec o;
+    o.kind = kind;
+    o.power = power;
+    if (kind == TDDA_OBJ_TARGETS) {
+        TDDA_REQUIRE(targets != nullptr && nt >= 1, tdda::ERR_INVALID, "targets required");
+        o.targets.assign(targets, targets + 3 * (size_t)nt);
+    }
+    return o;
+}
+
+int tdda_problem_objective(tdda_problem* p, int kind, int power, const double* targets, int nt, double* value,
+                           double* fields) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p && value, tdda::ERR_INVALID, "null argument");
+        std::vector<double2> f;
+        *value = tdda::objective_value(*p->p, obj_spec(kind, power, targets, nt), tdda::default_stream(),
+                                       fields ? &f : nullptr);
+        if (fields && !f.empty()) std::memcpy(fields, f.data(), f.size() * sizeof(double2));
+    });
+}
+
+int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* targets, int nt, int maxit,
+                          double tol, double* grad, int on_device, double* value, tdda_solve_info* adj) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p && grad, tdda::ERR_INVALID, "null argument");
+        tdda::SolveResult r = tdda::objective_gradient(*p->p, obj_spec(kind, power, targets, nt), maxit, tol, grad,
+                                                       on_device != 0, tdda::default_stream(), value);
+        if (adj) {
+            adj->status = r.status;
+            adj->iterations = r.iterations;
+            adj->error = r.err;
+            adj->reliable_updates = r.n_ru;
+            adj->ms = r.ms;
+        }
     });
 }
 

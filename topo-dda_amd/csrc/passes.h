@@ -23,8 +23,9 @@ enum XiMode : int {
 
 // Per-precision caps on pencils*length per tile.
 template <typename Real> struct Caps;
-template <> struct Caps<float> { static constexpr int XY = 4096; static constexpr int Z = 2048; };
-template <> struct Caps<double> { static constexpr int XY = 2048; static constexpr int Z = 1024; };
+// (the X pass holds three components per row, so 3*Lx <= X is required)
+template <> struct Caps<float> { static constexpr int X = 2048, Y = 4096, Z = 2048; };
+template <> struct Caps<double> { static constexpr int X = 1024, Y = 2048, Z = 1024; };
 
 // A launch over a batch: device arrays of jobs and of (job, tile) per block.
 struct LaunchSet {

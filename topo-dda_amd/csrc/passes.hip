@@ -83,17 +83,20 @@ __device__ inline bool publish_partials(const GridJob& J, int tile, int ntiles, 
     return s_last != 0;
 }
 
+// Called by every thread of the completing block: thread t sums tiles t,
+// t+256, ... and the block tree-reduces, a fixed order for a given job shape.
 template <int NV>
 __device__ inline void sum_partials(const GridJob& J, int ntiles, double (&out)[NV]) {
     __threadfence();
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = 0.0;
-    for (int t = 0; t < ntiles; ++t) {
+    for (int t = threadIdx.x; t < ntiles; t += FFT_THREADS) {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             out[k] += __hip_atomic_load(&J.partial[(long)t * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<NV>(out);
+    if (threadIdx.x == 0) __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ----------------------------------------------------------------------------
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
-    constexpr int CAP = Caps<Real>::XY;
+    constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_x[];
     C* lds = reinterpret_cast<C*>(smem_x);
     const int2 jt = bmap[blockIdx.x];
@@ -216,7 +219,7 @@ __global__ void __launch_bounds__(FFT_THREADS)
 k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
-    constexpr int CAP = Caps<Real>::XY;
+    constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_xi[];
     C* lds = reinterpret_cast<C*>(smem_xi);
     const int2 jt = bmap[blockIdx.x];
@@ -302,9 +305,9 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
     block_sum<3>(acc);
     const int ntiles = Tile<Real>::tx(J);
     if (!publish_partials<3>(J, jt.y, ntiles, acc)) return;
-    if (threadIdx.x != 0) return;
     double sum[3];
     sum_partials<3>(J, ntiles, sum);
+    if (threadIdx.x != 0) return;
     KScalars* ks = J.ks;
     const double2 dot = make_double2(sum[0], sum[1]);
     if (MODE == XI_V) {
@@ -326,7 +329,7 @@ template <typename Real, int DIR>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate) {
     using C = cx_t<Real>;
-    constexpr int CAP = Caps<Real>::XY;
+    constexpr int CAP = Caps<Real>::Y;
     extern __shared__ __align__(16) unsigned char smem_y[];
     C* lds = reinterpret_cast<C*>(smem_y);
     const int2 jt = bmap[blockIdx.x];
@@ -545,9 +548,9 @@ k_update(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int ga
     }
     block_sum<3>(acc);
     if (!publish_partials<3>(J, jt.y, J.tupd, acc)) return;
-    if (threadIdx.x != 0) return;
     double sum[3];
     sum_partials<3>(J, J.tupd, sum);
+    if (threadIdx.x != 0) return;
     const double2 rho_new = make_double2(sum[0], sum[1]);
     const double rn2 = sum[2];
     if (ks->refined) {
@@ -596,9 +599,9 @@ k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm
     }
     block_sum<2>(acc);
     if (!publish_partials<2>(J, jt.y, J.tupd, acc)) return;
-    if (threadIdx.x != 0) return;
     double sum[2];
     sum_partials<2>(J, J.tupd, sum);
+    if (threadIdx.x != 0) return;
     KScalars* ks = J.ks;
     ks->alpha = make_double2(0, 0);
     ks->beta = make_double2(0, 0);
@@ -740,11 +743,11 @@ void launch_init(const LaunchSet& ls, bool warm, hipStream_t st) {
 // ----------------------------------------------------------------------------
 void plan_tiles(GridJob& J) {
     auto cdiv = [](long a, long b) { return (int)((a + b - 1) / b); };
-    J.bx32 = std::max(1, Caps<float>::XY / (3 * J.lx));
-    J.by32 = std::max(1, Caps<float>::XY / J.ly);
+    J.bx32 = std::max(1, Caps<float>::X / (3 * J.lx));
+    J.by32 = std::max(1, Caps<float>::Y / J.ly);
     J.bz32 = std::max(1, Caps<float>::Z / J.lz);
-    J.bx64 = std::max(1, Caps<double>::XY / (3 * J.lx));
-    J.by64 = std::max(1, Caps<double>::XY / J.ly);
+    J.bx64 = std::max(1, Caps<double>::X / (3 * J.lx));
+    J.by64 = std::max(1, Caps<double>::Y / J.ly);
     J.bz64 = std::max(1, Caps<double>::Z / J.lz);
     const long nrows = (long)J.ny * J.nz;
     J.tx32 = cdiv(nrows, J.bx32);

@@ -48,6 +48,7 @@ class Problem {
     std::shared_ptr<Operator> op;
     std::unique_ptr<System> sys;
     DevBuf<double> rho;
+    SiciDevice tables;   // FCD tables (device), for field evaluations off the grid
     bool op_hit = false;
 };
 

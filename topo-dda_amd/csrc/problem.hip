@@ -104,7 +104,7 @@ OperatorKey problem_operator_key(const ProblemDesc& d, const SiciDevice& tables)
 
 Problem::Problem(const ProblemDesc& d, const SiciDevice& tables, bool refined, hipStream_t st,
                  std::shared_ptr<Operator> op_, std::unique_ptr<System> reuse)
-    : desc(d), op(std::move(op_)) {
+    : desc(d), op(std::move(op_)), tables(tables) {
     TDDA_REQUIRE(d.nx >= 1 && d.ny >= 1 && d.nz >= 1, ERR_INVALID, "empty box");
     TDDA_REQUIRE(d.method == 0 || d.method == 1, ERR_METHOD, "method must be FCD (0) or LDR (1)");
     TDDA_REQUIRE(d.lam > 0 && d.nback > 0 && d.d > 0, ERR_INVALID, "lambda, nback and d must be positive");

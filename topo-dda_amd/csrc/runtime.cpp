@@ -221,7 +221,7 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     ly = choose_fft_length(ny);
     lz = choose_fft_length(nz);
     TDDA_REQUIRE(lx <= MAX_LXY && ly <= MAX_LXY && lz <= MAX_LZ, ERR_INVALID,
-                 "grid too large for one device transform (Nx,Ny <= 256, Nz <= 128)");
+                 "grid too large for one device transform (Nx, Ny, Nz <= 128)");
     hx = lx / 2 + 1; hy = ly / 2 + 1; hz = lz / 2 + 1;
     PlanRegistry& reg = PlanRegistry::instance();
     px = reg.plan(lx);

@@ -68,7 +68,7 @@ template <typename T> void DevBuf<T>::release() {
 // ---------------------------------------------------------------------------
 // FFT lengths, plans and twiddle tables
 // ---------------------------------------------------------------------------
-constexpr int MAX_LXY = 512;   // limits set by the per-tile caps
+constexpr int MAX_LXY = 256;   // limits set by the per-tile caps (3*Lx <= X cap in FP64)
 constexpr int MAX_LZ = 256;
 
 int choose_fft_length(int n);             // smallest 2,3,5-smooth L >= max(2, 2n-1)

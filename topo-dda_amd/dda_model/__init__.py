@@ -14,6 +14,14 @@ import itertools
 import numpy as np
 from scipy.special import sici
 
+try:
+    # torch ships its own copy of the HIP runtime under the same soname; loading
+    # it before libtopodda.so keeps one HIP runtime in the process (otherwise the
+    # second copy to initialise finds no GPU).
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 from ._dda_model import DDAModel, device_bytes  # noqa: F401  (fails loudly if not built)
 
 __all__ = ["DDAModel", "DDAModelWrapper", "_generate_geometry", "sici_tables"]

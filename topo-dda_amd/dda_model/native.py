@@ -14,6 +14,7 @@ import numpy as np
 _LIB = None
 PREC = {"fp64": 0, "fp32_refined": 1}
 STATUS = {0: "active", 1: "converged", 2: "breakdown", 3: "maxit", 4: "idle"}
+OBJ = {"integrated": 0, "targets": 1}
 
 
 class ProblemDesc(ctypes.Structure):
@@ -65,6 +66,9 @@ def lib():
                                           ctypes.POINTER(SolveInfo)]
         L.tdda_problem_field.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
         L.tdda_problem_polarization.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.tdda_problem_objective.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp]
+        L.tdda_problem_gradient.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(SolveInfo)]
         L.tdda_problems_time_passes.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_server_create.argtypes = [ctypes.POINTER(ServerOpts), dp, dp, ctypes.c_long, ctypes.c_double,
                                          ctypes.POINTER(vp)]
@@ -75,6 +79,7 @@ def lib():
         L.tdda_server_idle.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tdda_device_bytes.restype = ctypes.c_long
+        L.tdda_trim_caches.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -157,6 +162,43 @@ class Problem:
         out = np.empty(3 * self.n, dtype=np.complex128)
         check(lib().tdda_problem_polarization(self.h, out.ctypes.data_as(ctypes.c_void_p), 0))
         return out
+
+    def solve(self, maxit=100000, tol=1e-5):
+        return solve([self], maxit, tol)[0]
+
+    @staticmethod
+    def _targets(targets):
+        if targets is None:
+            return None, 0
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 3)
+        return t, t.shape[0]
+
+    def objective(self, kind="integrated", power=2, targets=None, with_fields=False):
+        """Objective of the current solution: "integrated" = log sum |al P|^power,
+        "targets" = mean |E|^2 over external points (nm)."""
+        t, nt = self._targets(targets)
+        val = ctypes.c_double(0.0)
+        fields = np.empty(3 * nt, dtype=np.complex128) if (with_fields and nt) else None
+        check(lib().tdda_problem_objective(self.h, OBJ[kind], power, _dp(t) if t is not None else None, nt,
+                                           ctypes.byref(val),
+                                           _dp(fields.view(np.float64)) if fields is not None else None))
+        return (val.value, fields) if with_fields else val.value
+
+    def gradient(self, kind="integrated", power=2, targets=None, maxit=100000, tol=1e-5, out=None):
+        """d objective / d rho per voxel by one adjoint solve.  out: numpy array
+        (host) or torch CUDA float64 tensor (device); returns (grad, value, info)."""
+        t, nt = self._targets(targets)
+        val = ctypes.c_double(0.0)
+        info = SolveInfo()
+        if out is None:
+            out = np.empty(self.n, dtype=np.float64)
+        if hasattr(out, "data_ptr"):
+            ptr, dev = ctypes.cast(ctypes.c_void_p(out.data_ptr()), ctypes.POINTER(ctypes.c_double)), 1
+        else:
+            ptr, dev = _dp(out), 0
+        check(lib().tdda_problem_gradient(self.h, OBJ[kind], power, _dp(t) if t is not None else None, nt, maxit,
+                                          tol, ptr, dev, ctypes.byref(val), ctypes.byref(info)))
+        return out, val.value, dict(status=STATUS[info.status], iterations=info.iterations, error=info.error)
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
