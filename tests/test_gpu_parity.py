@@ -126,16 +126,47 @@ def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
     o.solveElectricField(p0, 100000, 1e-5)
     info = g.solveInfo(0)
     _log(f"[gpu] iterations gpu {info['iterations']} oracle {o.info.iterations}")
-    assert abs(info["iterations"] - o.info.iterations) <= 1
-    # two solves that each stop at a 1e-5 residual differ by about the error of
-    # either (the rounding of the FFTs steers the Krylov trajectory); the
-    # tight comparison below is the parity statement
+    # this case's residual plateaus near 1e-5: the oracle alone stops anywhere
+    # in 40..46 iterations when only the FFT grid (hence the rounding) changes,
+    # so the count is not a parity quantity here (it is checked on a case with
+    # a stable count in test_iteration_count_matches_oracle). Two solves that
+    # each stop at a 1e-5 residual differ by about the error of either; the
+    # tight comparison below is the parity statement.
+    assert info["status"] == "converged" and info["error"] <= 1e-5
+    assert 30 <= info["iterations"] <= 60
     assert _rel(g.getElectricField(), o.getElectricField()) < 5e-5
     assert g.calculateObjective() == pytest.approx(o.calculateObjective(), rel=1e-5)
     # tightly converged: both solve the same system
     g.solveElectricField(p0, 100000, 1e-11)
     o.solveElectricField(p0, 100000, 1e-11)
     assert _rel(g.getElectricField(), o.getElectricField()) < 1e-9
+
+
+@pytest.mark.parametrize("name,case", [CASES[1], CASES[2], CASES[3]], ids=[CASES[i][0] for i in (1, 2, 3)])
+def test_iteration_count_matches_oracle(dm, O, tables, name, case):
+    """Same BiCGSTAB recurrence and stopping rule (DDAModel.cpp:899-1009).
+
+    The count at which the recursive residual first drops below tol moves by
+    a few iterations under rounding alone: the oracle on three FFT grids (the
+    same operator, rounded differently) gives e.g. 100..105 for the 16^3 case.
+    The GPU count must fall inside the oracle's own spread (+-2); a wrong
+    recurrence (a sign, a swapped omega, a stale rho) moves it far outside or
+    stops convergence."""
+    _log(f"[gpu] iteration count {name}")
+    a = _args(O, tables, **case)
+    nx, ny, nz = case["shape"]
+    counts = []
+    for fs in (None, (2 * nx, 2 * ny, 2 * nz), (2 * nx + 2, 2 * ny + 2, 2 * nz + 2)):
+        o = O.OracleDDAModel(*a, fft_shape=fs)
+        o.solveElectricField(np.zeros(3 * o.N, complex), 100000, 1e-6)
+        counts.append(o.info.iterations)
+    g = dm.DDAModel(*a)
+    g.solveElectricField(np.zeros(3 * o.N, complex), 100000, 1e-6)
+    info = g.solveInfo(0)
+    _log(f"[gpu] iterations gpu {info['iterations']} oracle {counts}")
+    assert info["status"] == "converged"
+    assert min(counts) - 2 <= info["iterations"] <= max(counts) + 2
+    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
 
 
 def test_refined_solve_accuracy(dm, O, tables):
