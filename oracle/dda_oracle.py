@@ -317,7 +317,9 @@ class SolveInfo:
     matvecs: int = 0
 
 
-def bicgstab_reference(apply, E, P0, maxit, tol):
+def bicgstab_reference(apply, E, P0, maxit, tol, history=None):
+    """BiCGSTAB of DDAModel.cpp:899-1009; history (a list) receives the
+    relative recursive residual after every loop step."""
     P = np.array(P0, dtype=np.complex128, copy=True)
     info = SolveInfo()
     nE = np.linalg.norm(E)
@@ -355,6 +357,8 @@ def bicgstab_reference(apply, E, P0, maxit, tol):
         rl = r
         r = t - zeta * s
         info.matvecs += 2
+        if history is not None:
+            history.append(float(np.linalg.norm(r) / nE))
         if np.linalg.norm(r) / nE <= tol:
             info.error = np.linalg.norm(r) / nE
             info.iterations = it + 1

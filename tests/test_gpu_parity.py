@@ -126,12 +126,11 @@ def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
     o.solveElectricField(p0, 100000, 1e-5)
     info = g.solveInfo(0)
     _log(f"[gpu] iterations gpu {info['iterations']} oracle {o.info.iterations}")
-    # this case's residual plateaus near 1e-5: the oracle alone stops anywhere
-    # in 40..46 iterations when only the FFT grid (hence the rounding) changes,
-    # so the count is not a parity quantity here (it is checked on a case with
-    # a stable count in test_iteration_count_matches_oracle). Two solves that
-    # each stop at a 1e-5 residual differ by about the error of either; the
-    # tight comparison below is the parity statement.
+    # the oracle alone stops anywhere in 40..46 iterations here when only the
+    # FFT grid (hence the rounding) changes, so the count is not a parity
+    # quantity (the recurrence is pinned in test_residual_history_matches_oracle).
+    # Two solves that each stop at a 1e-5 residual differ by about the error
+    # of either; the tight comparison below is the parity statement.
     assert info["status"] == "converged" and info["error"] <= 1e-5
     assert 30 <= info["iterations"] <= 60
     assert _rel(g.getElectricField(), o.getElectricField()) < 5e-5
@@ -143,29 +142,35 @@ def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
 
 
 @pytest.mark.parametrize("name,case", [CASES[1], CASES[2], CASES[3]], ids=[CASES[i][0] for i in (1, 2, 3)])
-def test_iteration_count_matches_oracle(dm, O, tables, name, case):
+def test_residual_history_matches_oracle(dm, O, tables, name, case):
     """Same BiCGSTAB recurrence and stopping rule (DDAModel.cpp:899-1009).
 
-    The count at which the recursive residual first drops below tol moves by
-    a few iterations under rounding alone: the oracle on three FFT grids (the
-    same operator, rounded differently) gives e.g. 100..105 for the 16^3 case.
-    The GPU count must fall inside the oracle's own spread (+-2); a wrong
-    recurrence (a sign, a swapped omega, a stale rho) moves it far outside or
-    stops convergence."""
-    _log(f"[gpu] iteration count {name}")
+    The recurrence amplifies rounding about tenfold per step: the oracle on
+    two FFT grids (the same operator, rounded differently) agrees to ~1e-11
+    after 8 steps but only to ~1e-1 after 30, so the iteration count at which
+    the residual first drops below tol moves by several steps under rounding
+    alone (66..74 for the 7x9x11 case).  The recurrence itself is pinned by
+    the residual after each of the first steps (maxit = k reports it), which
+    a wrong scalar (a sign, a swapped omega, a stale rho) changes at O(1);
+    the converged field and a loose count window check the rest."""
+    _log(f"[gpu] residual history {name}")
     a = _args(O, tables, **case)
-    nx, ny, nz = case["shape"]
-    counts = []
-    for fs in (None, (2 * nx, 2 * ny, 2 * nz), (2 * nx + 2, 2 * ny + 2, 2 * nz + 2)):
-        o = O.OracleDDAModel(*a, fft_shape=fs)
-        o.solveElectricField(np.zeros(3 * o.N, complex), 100000, 1e-6)
-        counts.append(o.info.iterations)
+    o = O.OracleDDAModel(*a)
+    hist = []
+    O.bicgstab_reference(o.apply, o.E, np.zeros(3 * o.N, complex), 100000, 1e-6, history=hist)
     g = dm.DDAModel(*a)
-    g.solveElectricField(np.zeros(3 * o.N, complex), 100000, 1e-6)
+    p0 = np.zeros(3 * o.N, complex)
+    for k in (1, 2, 3, 4, 6, 8):
+        g.solveElectricField(p0, k, 1e-30)
+        info = g.solveInfo(0)
+        assert info["status"] == "maxit" and info["iterations"] == k
+        assert info["error"] == pytest.approx(hist[k - 1], rel=1e-7), k
+    g.solveElectricField(p0, 100000, 1e-6)
     info = g.solveInfo(0)
-    _log(f"[gpu] iterations gpu {info['iterations']} oracle {counts}")
-    assert info["status"] == "converged"
-    assert min(counts) - 2 <= info["iterations"] <= max(counts) + 2
+    _log(f"[gpu] iterations gpu {info['iterations']} oracle {len(hist)}")
+    assert info["status"] == "converged" and info["error"] <= 1e-6
+    assert 0.8 * len(hist) <= info["iterations"] <= 1.25 * len(hist)
+    o.solveElectricField(p0, 100000, 1e-6)
     assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
 
 

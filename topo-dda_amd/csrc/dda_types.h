@@ -48,6 +48,8 @@ struct GridJob {
     int tx32, ty32, tz32;      // tiles per pass, c64
     int tx64, ty64, tz64;      // tiles per pass, c128
     int tupd;                  // tiles of the elementwise update kernel
+    int fx, fy, fz;            // 1: the pass uses the length-specialised kernels
+    unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients
     const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
     // operator data
     const float2* green32;     // [6][hz][hy][hx]

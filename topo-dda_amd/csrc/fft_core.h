@@ -33,7 +33,7 @@ struct FftPlan {
 template <int R, int DIR, typename C> struct Dft;
 
 template <int DIR, typename C> struct Dft<2, DIR, C> {
-    __device__ static inline void run(C* v) {
+    __host__ __device__ static inline void run(C* v) {
         C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -41,7 +41,7 @@ template <int DIR, typename C> struct Dft<2, DIR, C> {
 };
 
 template <int DIR, typename C> struct Dft<3, DIR, C> {
-    __device__ static inline void run(C* v) {
+    __host__ __device__ static inline void run(C* v) {
         using R = decltype(v[0].x);
         const R h = (R)0.8660254037844386467637231707529362;  // sqrt(3)/2
         C a = v[0];
@@ -56,7 +56,7 @@ template <int DIR, typename C> struct Dft<3, DIR, C> {
 };
 
 template <int DIR, typename C> struct Dft<4, DIR, C> {
-    __device__ static inline void run(C* v) {
+    __host__ __device__ static inline void run(C* v) {
         C t0 = cadd(v[0], v[2]);
         C t1 = csub(v[0], v[2]);
         C t2 = cadd(v[1], v[3]);
@@ -69,7 +69,7 @@ template <int DIR, typename C> struct Dft<4, DIR, C> {
 };
 
 template <int DIR, typename C> struct Dft<5, DIR, C> {
-    __device__ static inline void run(C* v) {
+    __host__ __device__ static inline void run(C* v) {
         using R = decltype(v[0].x);
         const R c1 = (R)0.3090169943749474241022934171828191;   // cos(2pi/5)
         const R c2 = (R)-0.8090169943749474241022934171828191;  // cos(4pi/5)
@@ -91,7 +91,7 @@ template <int DIR, typename C> struct Dft<5, DIR, C> {
 };
 
 template <int DIR, typename C> struct Dft<8, DIR, C> {
-    __device__ static inline void run(C* v) {
+    __host__ __device__ static inline void run(C* v) {
         using R = decltype(v[0].x);
         const R h = (R)0.7071067811865475244008443621048490;  // sqrt(2)/2
         C e[4] = {v[0], v[2], v[4], v[6]};

@@ -1,0 +1,383 @@
+// Length-specialised pencil passes (L = L1 * L2 with both factors fixed at
+// compile time).  Each pass is split into per-task functions: the kernels
+// call them in block-strided loops separated by barriers, and a CPU harness
+// can call them in plain loops, so indexing and arithmetic are checkable
+// without a GPU.
+//
+// Layout of the LDS exchange between step A (task = (pencil, n2), inner
+// L1-point DFT over n1, twiddle w_L^(n2 k1)) and step B (task = (pencil, k1),
+// L2-point DFT over n2):
+//   Y, Z : [component][k1*L2 + n2][pencil]   (pencil fastest; pencils are
+//          consecutive x columns, so global accesses are coalesced across
+//          the pencil index)
+//   X    : [pencil][k1*(L2+1) + n2] with an odd pencil stride (rows are
+//          contiguous in memory, so global accesses are coalesced across
+//          positions instead).
+#pragma once
+
+#include "fft_small.h"
+
+#ifndef TDDA_HD
+#define TDDA_HD __host__ __device__ __forceinline__
+#endif
+
+namespace tdda {
+
+// twiddle w_L^(j) from the table (forward sign), conjugated for DIR > 0
+template <int DIR, typename C>
+TDDA_HD C tw_at(const C* __restrict__ tw, int j) {
+    C w = tw[j];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+// Pencils / rows per tile: step-A task counts stay at or below one pass of
+// a 256-thread block (FP64 tiles hold half as many).
+template <typename Real> TDDA_HD constexpr int fast_np_y(int L1, int L2) {
+    return (256 / L2) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L1;
+}
+template <typename Real> TDDA_HD constexpr int fast_np_z(int L1, int L2) {
+    return (256 / L1) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L2;
+}
+template <typename Real> TDDA_HD constexpr int fast_nr_x(int L1, int L2) {
+    return (256 / (3 * L2)) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L1;
+}
+TDDA_HD constexpr int fast_ps_x(int L1, int L2) { return (L1 * (L2 + 1)) | 1; }
+
+// 32-bit quotient a / d by a precomputed m = ceil(2^32 / d) (exact for a*d < 2^32)
+TDDA_HD unsigned fdiv(unsigned a, unsigned long long m) {
+    return (unsigned)(((unsigned long long)a * m) >> 32);
+}
+
+// ---------------------------------------------------------------------------
+// Y pass: np pencils along y, positions pos in [0, L), global address of
+// (pencil b, pos) supplied by the caller.
+// ---------------------------------------------------------------------------
+template <int L1, int L2, int NP, int DIR, typename C>
+struct YFast {
+    static constexpr int L = L1 * L2;
+    static constexpr int TA = NP * L2;
+    static constexpr int TB = NP * L1;
+    static constexpr int LDS = NP * L;   // complex entries (+ L for twiddles)
+
+    template <typename Load>
+    TDDA_HD static void step_a(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, int n_in, Load load) {
+        const int b = tau % NP;
+        const int n2 = tau / NP;
+        if (b >= nb) return;
+        C v[L1];
+        bool done = false;
+        if constexpr (L1 % 2 == 0) {
+            if (2 * n_in <= L) {
+                static_for<L1 / 2>([&](auto i) {
+                    constexpr int n1 = decltype(i)::value;
+                    const int pos = n2 + L2 * n1;
+                    v[n1] = pos < n_in ? load(b, pos) : mk<C>(0.0, 0.0);
+                });
+                dft_half_in<L1, DIR>(v);
+                done = true;
+            }
+        }
+        if (!done) {
+            static_for<L1>([&](auto i) {
+                constexpr int n1 = decltype(i)::value;
+                const int pos = n2 + L2 * n1;
+                v[n1] = pos < n_in ? load(b, pos) : mk<C>(0.0, 0.0);
+            });
+            SDft<L1, DIR, C>::run(v);
+        }
+        static_for<L1>([&](auto i) {
+            constexpr int k1 = decltype(i)::value;
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<DIR>(tw, n2 * k1));
+            lds[(k1 * L2 + n2) * NP + b] = v[k1];
+        });
+    }
+
+    template <typename Store>
+    TDDA_HD static void step_b(int tau, const C* __restrict__ lds, int nb, int n_out, Store store) {
+        const int b = tau % NP;
+        const int k1 = tau / NP;
+        if (b >= nb) return;
+        C v[L2];
+        static_for<L2>([&](auto i) {
+            constexpr int n2 = decltype(i)::value;
+            v[n2] = lds[(k1 * L2 + n2) * NP + b];
+        });
+        if constexpr (L2 % 2 == 0) {
+            if (2 * n_out <= L) {
+                dft_half_out<L2, DIR>(v);
+                static_for<L2 / 2>([&](auto i) {
+                    constexpr int k2 = decltype(i)::value;
+                    const int pos = k1 + L1 * k2;
+                    if (pos < n_out) store(b, pos, v[k2]);
+                });
+                return;
+            }
+        }
+        SDft<L2, DIR, C>::run(v);
+        static_for<L2>([&](auto i) {
+            constexpr int k2 = decltype(i)::value;
+            const int pos = k1 + L1 * k2;
+            if (pos < n_out) store(b, pos, v[k2]);
+        });
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Z pass: forward transform of three components, 3x3 multiply per spectral
+// point, inverse transform; input and output positions < n (pruned).
+// ---------------------------------------------------------------------------
+template <int L1, int L2, int NP, typename C>
+struct ZFast {
+    static constexpr int L = L1 * L2;
+    static constexpr int TA = 3 * NP * L2;
+    static constexpr int TB = NP * L1;
+    static constexpr int LDS = 3 * NP * L;
+
+    TDDA_HD static int at(int c, int k1, int n2, int b) { return ((c * L + k1 * L2 + n2) * NP) + b; }
+
+    // forward, step A: load(c, b, pos)
+    template <typename Load>
+    TDDA_HD static void fwd_a(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, int n, Load load) {
+        const int b = tau % NP;
+        const int rest = tau / NP;
+        const int n2 = rest % L2;
+        const int c = rest / L2;
+        if (b >= nb) return;
+        C v[L1];
+        bool done = false;
+        if constexpr (L1 % 2 == 0) {
+            if (2 * n <= L) {
+                static_for<L1 / 2>([&](auto i) {
+                    constexpr int n1 = decltype(i)::value;
+                    const int pos = n2 + L2 * n1;
+                    v[n1] = pos < n ? load(c, b, pos) : mk<C>(0.0, 0.0);
+                });
+                dft_half_in<L1, -1>(v);
+                done = true;
+            }
+        }
+        if (!done) {
+            static_for<L1>([&](auto i) {
+                constexpr int n1 = decltype(i)::value;
+                const int pos = n2 + L2 * n1;
+                v[n1] = pos < n ? load(c, b, pos) : mk<C>(0.0, 0.0);
+            });
+            SDft<L1, -1, C>::run(v);
+        }
+        static_for<L1>([&](auto i) {
+            constexpr int k1 = decltype(i)::value;
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<-1>(tw, n2 * k1));
+            lds[at(c, k1, n2, b)] = v[k1];
+        });
+    }
+
+    // step B forward + multiply (mul(b, kz, v0, v1, v2)) + step B inverse,
+    // in place: the task reads and writes the same LDS entries.
+    template <typename Mul>
+    TDDA_HD static void mid_b(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Mul mul) {
+        const int b = tau % NP;
+        const int k1 = tau / NP;
+        if (b >= nb) return;
+        C v[3][L2];
+        static_for<3>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            static_for<L2>([&](auto i) {
+                constexpr int n2 = decltype(i)::value;
+                v[c][n2] = lds[at(c, k1, n2, b)];
+            });
+            SDft<L2, -1, C>::run(v[c]);
+        });
+        static_for<L2>([&](auto i) {
+            constexpr int k2 = decltype(i)::value;
+            mul(b, k1 + L1 * k2, v[0][k2], v[1][k2], v[2][k2]);
+        });
+        static_for<3>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            SDft<L2, 1, C>::run(v[c]);
+            static_for<L2>([&](auto i) {
+                constexpr int n2 = decltype(i)::value;
+                C w = v[c][n2];
+                if constexpr (n2 > 0) w = cmul(w, tw_at<1>(tw, n2 * k1));
+                lds[at(c, k1, n2, b)] = w;
+            });
+        });
+    }
+
+    // inverse, step A: store(c, b, pos, value) for pos < n
+    template <typename Store>
+    TDDA_HD static void inv_a(int tau, const C* __restrict__ lds, int nb, int n, Store store) {
+        const int b = tau % NP;
+        const int rest = tau / NP;
+        const int n2 = rest % L2;
+        const int c = rest / L2;
+        if (b >= nb) return;
+        C v[L1];
+        static_for<L1>([&](auto i) {
+            constexpr int k1 = decltype(i)::value;
+            v[k1] = lds[at(c, k1, n2, b)];
+        });
+        if constexpr (L1 % 2 == 0) {
+            if (2 * n <= L) {
+                dft_half_out<L1, 1>(v);
+                static_for<L1 / 2>([&](auto i) {
+                    constexpr int n1 = decltype(i)::value;
+                    const int pos = n2 + L2 * n1;
+                    if (pos < n) store(c, b, pos, v[n1]);
+                });
+                return;
+            }
+        }
+        SDft<L1, 1, C>::run(v);
+        static_for<L1>([&](auto i) {
+            constexpr int n1 = decltype(i)::value;
+            const int pos = n2 + L2 * n1;
+            if (pos < n) store(c, b, pos, v[n1]);
+        });
+    }
+};
+
+// ---------------------------------------------------------------------------
+// X pass: NR rows of three components (pencil b = 3*row + c) along x.
+// ---------------------------------------------------------------------------
+template <int L1, int L2, int NR, typename C>
+struct XFast {
+    static constexpr int L = L1 * L2;
+    static constexpr int PS = fast_ps_x(L1, L2);
+    static constexpr int TA = 3 * NR * L2;
+    static constexpr int TB = 3 * NR * L1;
+    static constexpr int LDS = 3 * NR * PS;
+
+    TDDA_HD static int at(int b, int k1, int n2) { return b * PS + k1 * (L2 + 1) + n2; }
+
+    // forward A: task (c fastest, n2, row); load(row, c, x) for x < n_in
+    template <typename Load>
+    TDDA_HD static void fwd_a(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nr, int n_in, Load load) {
+        const int c = tau % 3;
+        const int rest = tau / 3;
+        const int n2 = rest % L2;
+        const int r = rest / L2;
+        if (r >= nr) return;
+        C v[L1];
+        bool done = false;
+        if constexpr (L1 % 2 == 0) {
+            if (2 * n_in <= L) {
+                static_for<L1 / 2>([&](auto i) {
+                    constexpr int n1 = decltype(i)::value;
+                    const int x = n2 + L2 * n1;
+                    v[n1] = x < n_in ? load(r, c, x) : mk<C>(0.0, 0.0);
+                });
+                dft_half_in<L1, -1>(v);
+                done = true;
+            }
+        }
+        if (!done) {
+            static_for<L1>([&](auto i) {
+                constexpr int n1 = decltype(i)::value;
+                const int x = n2 + L2 * n1;
+                v[n1] = x < n_in ? load(r, c, x) : mk<C>(0.0, 0.0);
+            });
+            SDft<L1, -1, C>::run(v);
+        }
+        const int b = 3 * r + c;
+        static_for<L1>([&](auto i) {
+            constexpr int k1 = decltype(i)::value;
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<-1>(tw, n2 * k1));
+            lds[at(b, k1, n2)] = v[k1];
+        });
+    }
+
+    // forward B: task (k1 fastest, pencil); store(row, c, x, value), all x < L
+    template <typename Store>
+    TDDA_HD static void fwd_b(int tau, const C* __restrict__ lds, int nr, Store store) {
+        const int k1 = tau % L1;
+        const int b = tau / L1;
+        if (b >= 3 * nr) return;
+        C v[L2];
+        static_for<L2>([&](auto i) {
+            constexpr int n2 = decltype(i)::value;
+            v[n2] = lds[at(b, k1, n2)];
+        });
+        SDft<L2, -1, C>::run(v);
+        const int r = b / 3;
+        const int c = b - 3 * r;
+        static_for<L2>([&](auto i) {
+            constexpr int k2 = decltype(i)::value;
+            store(r, c, k1 + L1 * k2, v[k2]);
+        });
+    }
+
+    // inverse A: task (n2 fastest, pencil); load(row, c, x) for all x < L
+    template <typename Load>
+    TDDA_HD static void inv_a(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nr, Load load) {
+        const int n2 = tau % L2;
+        const int b = tau / L2;
+        if (b >= 3 * nr) return;
+        const int r = b / 3;
+        const int c = b - 3 * r;
+        C v[L1];
+        static_for<L1>([&](auto i) {
+            constexpr int n1 = decltype(i)::value;
+            v[n1] = load(r, c, n2 + L2 * n1);
+        });
+        SDft<L1, 1, C>::run(v);
+        static_for<L1>([&](auto i) {
+            constexpr int k1 = decltype(i)::value;
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<1>(tw, n2 * k1));
+            lds[at(b, k1, n2)] = v[k1];
+        });
+    }
+
+    // inverse B: task (c fastest, k1, row); epi(row, c, x, value) for x < n_out
+    template <typename Epi>
+    TDDA_HD static void inv_b(int tau, const C* __restrict__ lds, int nr, int n_out, Epi epi) {
+        const int c = tau % 3;
+        const int rest = tau / 3;
+        const int k1 = rest % L1;
+        const int r = rest / L1;
+        if (r >= nr) return;
+        const int b = 3 * r + c;
+        C v[L2];
+        static_for<L2>([&](auto i) {
+            constexpr int n2 = decltype(i)::value;
+            v[n2] = lds[at(b, k1, n2)];
+        });
+        if constexpr (L2 % 2 == 0) {
+            if (2 * n_out <= L) {
+                dft_half_out<L2, 1>(v);
+                static_for<L2 / 2>([&](auto i) {
+                    constexpr int k2 = decltype(i)::value;
+                    const int x = k1 + L1 * k2;
+                    if (x < n_out) epi(r, c, x, v[k2]);
+                });
+                return;
+            }
+        }
+        SDft<L2, 1, C>::run(v);
+        static_for<L2>([&](auto i) {
+            constexpr int k2 = decltype(i)::value;
+            const int x = k1 + L1 * k2;
+            if (x < n_out) epi(r, c, x, v[k2]);
+        });
+    }
+};
+
+// Call f(IC<L1>, IC<L2>) for a length with a specialised split; false if none.
+template <typename F>
+TDDA_HD bool with_split(int L, F&& f) {
+    switch (L) {
+        case 16: f(IC<4>{}, IC<4>{}); return true;
+        case 20: f(IC<5>{}, IC<4>{}); return true;
+        case 24: f(IC<6>{}, IC<4>{}); return true;
+        case 32: f(IC<8>{}, IC<4>{}); return true;
+        case 40: f(IC<8>{}, IC<5>{}); return true;
+        case 48: f(IC<8>{}, IC<6>{}); return true;
+        case 64: f(IC<8>{}, IC<8>{}); return true;
+        case 80: f(IC<10>{}, IC<8>{}); return true;
+        case 96: f(IC<12>{}, IC<8>{}); return true;
+        case 128: f(IC<16>{}, IC<8>{}); return true;
+        default: return false;
+    }
+}
+
+}  // namespace tdda

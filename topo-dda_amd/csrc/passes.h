@@ -27,17 +27,24 @@ template <typename Real> struct Caps;
 template <> struct Caps<float> { static constexpr int X = 2048, Y = 4096, Z = 2048; };
 template <> struct Caps<double> { static constexpr int X = 1024, Y = 2048, Z = 1024; };
 
-// A launch over a batch: device arrays of jobs and of (job, tile) per block.
+// One launch of one pass: device map of (job, tile) per block, its length,
+// and the dynamic LDS bytes (the largest tile of any job in the map).
+struct PassMap {
+    const int2* map = nullptr;
+    int n = 0;
+    size_t lds = 0;
+};
+enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
+
+// A launch over a batch: device array of jobs and, per transform precision
+// and pass, the blocks of the generic kernels and of the length-specialised
+// kernels (a job uses the specialised kernel of a pass when that pass's
+// length has a compile-time split).
 struct LaunchSet {
     const GridJob* jobs = nullptr;   // device
-    const int2* map_x32 = nullptr; int n_x32 = 0;
-    const int2* map_y32 = nullptr; int n_y32 = 0;
-    const int2* map_z32 = nullptr; int n_z32 = 0;
-    const int2* map_x64 = nullptr; int n_x64 = 0;
-    const int2* map_y64 = nullptr; int n_y64 = 0;
-    const int2* map_z64 = nullptr; int n_z64 = 0;
-    const int2* map_upd = nullptr; int n_upd = 0;
-    size_t lds_x32 = 0, lds_y32 = 0, lds_z32 = 0, lds_x64 = 0, lds_y64 = 0, lds_z64 = 0;
+    PassMap pm[2][3][2];             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
+    const int2* map_upd = nullptr;
+    int n_upd = 0;
     bool refined = false;            // Krylov vectors are c64
 };
 
@@ -60,11 +67,23 @@ void launch_update(const LaunchSet& ls, int gate, hipStream_t st);
 // Initial residual, r0, p = v = 0 and scalars; warm: r = rhs - yout.
 void launch_init(const LaunchSet& ls, bool warm, hipStream_t st);
 
-// Host-side tile geometry for a job (fills the b*/t* fields).
+// Host-side tile geometry for a job (fills the b*/t*/f* fields).
 void plan_tiles(GridJob& J);
-size_t lds_bytes_x(const GridJob& J, bool c64);
-size_t lds_bytes_y(const GridJob& J, bool c64);
-size_t lds_bytes_z(const GridJob& J, bool c64);
+// Dynamic LDS bytes of the job's tile in a pass (generic or specialised).
+size_t lds_bytes(const GridJob& J, int pass, bool c64);
+
+// Length-specialised kernels (passes_fast.hip).
+void launch_fast_xfwd(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
+                      int mode, int gate, hipStream_t st);
+void launch_fast_xinv(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
+                      int mode, int gate, hipStream_t st);
+void launch_fast_y(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int dir, int gate,
+                   hipStream_t st);
+void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int gate,
+                   hipStream_t st);
+// Rows (X) or pencils (Y, Z) per tile and LDS bytes of the specialised path
+// for length L; false if L has no specialised split.
+bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes);
 constexpr int UPD_EPT = 4;   // elements per thread in the update kernel
 
 }  // namespace tdda

@@ -15,122 +15,9 @@
 // launch covers a batch of jobs; block b handles (job, tile) = map[b].  Tiles
 // depend only on a job's own shape, and reductions are per job in a fixed
 // order, so a job's result does not depend on what else is in the batch.
-#include "passes.h"
+#include "pass_util.h"
 
 namespace tdda {
-
-// ----------------------------------------------------------------------------
-// small helpers
-// ----------------------------------------------------------------------------
-template <typename Real> struct Tile;
-template <> struct Tile<float> {
-    __device__ static int bx(const GridJob& J) { return J.bx32; }
-    __device__ static int by(const GridJob& J) { return J.by32; }
-    __device__ static int bz(const GridJob& J) { return J.bz32; }
-    __device__ static int tx(const GridJob& J) { return J.tx32; }
-    __device__ static const float2* green(const GridJob& J) { return J.green32; }
-    __device__ static const float2* al(const GridJob& J) { return J.al32; }
-};
-template <> struct Tile<double> {
-    __device__ static int bx(const GridJob& J) { return J.bx64; }
-    __device__ static int by(const GridJob& J) { return J.by64; }
-    __device__ static int bz(const GridJob& J) { return J.bz64; }
-    __device__ static int tx(const GridJob& J) { return J.tx64; }
-    __device__ static const double2* green(const GridJob& J) { return J.green64; }
-    __device__ static const double2* al(const GridJob& J) { return J.al64; }
-};
-
-__device__ inline bool gate_open(const GridJob& J, int gate) {
-    if (gate == GATE_ALWAYS) return true;
-    const KScalars* ks = J.ks;
-    if (ks->status != JOB_ACTIVE) return false;
-    if (gate == GATE_RU) return ks->need_ru != 0;
-    return true;
-}
-
-// Deterministic block reduction of NV doubles; thread 0 gets the sums.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV]) {
-    __shared__ double red[NV][FFT_THREADS];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (int s = FFT_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = red[k][0];
-}
-
-// Publish this block's partial sums; returns true (in every thread) for the
-// block that completes the job.  That block then sums all partials in tile
-// order (fixed, hence deterministic) through sum_partials().
-template <int NV>
-__device__ inline bool publish_partials(const GridJob& J, int tile, int ntiles, const double (&v)[NV]) {
-    __shared__ int s_last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) J.partial[(long)tile * 6 + k] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(J.counter, 1u);
-        s_last = (prev + 1 == (unsigned)ntiles) ? 1 : 0;
-    }
-    __syncthreads();
-    return s_last != 0;
-}
-
-// Called by every thread of the completing block: thread t sums tiles t,
-// t+256, ... and the block tree-reduces, a fixed order for a given job shape.
-template <int NV>
-__device__ inline void sum_partials(const GridJob& J, int ntiles, double (&out)[NV]) {
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = 0.0;
-    for (int t = threadIdx.x; t < ntiles; t += FFT_THREADS) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            out[k] += __hip_atomic_load(&J.partial[(long)t * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(out);
-    if (threadIdx.x == 0) __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ----------------------------------------------------------------------------
-// scalar logic (runs in one thread of the last block), DDAModel.cpp:945-1004
-// ----------------------------------------------------------------------------
-__device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2) {
-    ks->rnorm2 = rn2;
-    const double rel = sqrt(rn2 / ks->enorm2);
-    const int it = ks->iter;
-    if (it >= 0 && rel <= ks->tol) {      // ||r||/||E|| <= tol after a loop step
-        ks->status = JOB_CONVERGED;
-        ks->iterations = it + 1;
-        ks->err = rel;
-        return;
-    }
-    const int next = it + 1;
-    if (next >= ks->maxit) {
-        ks->status = JOB_MAXIT;
-        ks->iterations = ks->maxit;
-        ks->err = rel;
-        return;
-    }
-    if (ks->rho.x == 0.0 && ks->rho.y == 0.0) {   // r0.dot(rl) == 0 at the top of the loop
-        ks->status = JOB_BREAKDOWN;
-        ks->iterations = next + 1;
-        ks->err = rel;
-        return;
-    }
-    // beta = (alpha/zeta) * <r0,r> / <r0,rl>
-    ks->beta = ddiv(dcmul(ddiv(ks->alpha, ks->zeta), rho_new), ks->rho);
-    ks->rho_prev = ks->rho;
-    ks->rho = rho_new;
-    ks->iter = next;
-}
 
 // ----------------------------------------------------------------------------
 // X forward pass
@@ -154,45 +41,14 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
     const int np = 3 * nb;
     const int bp = np | 1;
     const int nel = 3 * nx * nb;
-    double2 alpha = make_double2(0, 0), beta = make_double2(0, 0), zeta = make_double2(0, 0);
-    if (MODE == XF_P) { beta = J.ks->beta; zeta = J.ks->zeta; }
-    if (MODE == XF_T) { alpha = J.ks->alpha; }
+    const XfScalars sc = xf_scalars<MODE>(J);
     for (int e = threadIdx.x; e < nel; e += FFT_THREADS) {
         const int c = e % 3;
         const int rem = e / 3;
         const int x = rem % nx;
         const int b = rem / nx;
-        long vi;
-        if (J.box_map == nullptr) {
-            vi = 3L * nx * q0 + e;
-        } else {
-            const int dip = J.box_map[(long)(q0 + b) * nx + x];
-            vi = dip >= 0 ? 3L * dip + c : -1;
-        }
-        C u = mk<C>(0.0, 0.0);
-        if (vi >= 0) {
-            if (MODE == XF_PLAIN) {
-                u = static_cast<const C*>(J.xin)[vi];
-            } else if (MODE == XF_SOL) {
-                u = from_d<C>(J.P[vi]);
-            } else if (MODE == XF_P) {
-                KC* r = static_cast<KC*>(J.r);
-                KC* p = static_cast<KC*>(J.p);
-                KC* v = static_cast<KC*>(J.v);
-                const double2 w = csub(to_d(p[vi]), dcmul(zeta, to_d(v[vi])));   // p - u, u = zeta v
-                const KC pk = from_d<KC>(cadd(to_d(r[vi]), dcmul(beta, w)));
-                p[vi] = pk;
-                u = from_d<C>(to_d(pk));
-            } else if (MODE == XF_T) {
-                KC* r = static_cast<KC*>(J.r);
-                KC* v = static_cast<KC*>(J.v);
-                KC* t = static_cast<KC*>(J.t);
-                const KC tk = from_d<KC>(csub(to_d(r[vi]), dcmul(alpha, to_d(v[vi]))));
-                t[vi] = tk;
-                u = from_d<C>(to_d(tk));
-            }
-        }
-        lds[x * bp + 3 * b + c] = u;
+        const long vi = vec_index(J, q0 + b, x, c);
+        lds[x * bp + 3 * b + c] = xf_element<C, KC, MODE>(J, vi, sc);
     }
     __syncthreads();
     lds_fft<-1, CAP, 1>(J.px, lds, bp, np, nx, tw);
@@ -257,69 +113,11 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
         const int rem = e / 3;
         const int x = rem % nx;
         const int b = rem / nx;
-        long vi;
-        if (J.box_map == nullptr) {
-            vi = 3L * nx * q0 + e;
-        } else {
-            const int dip = J.box_map[(long)(q0 + b) * nx + x];
-            vi = dip >= 0 ? 3L * dip + c : -1;
-        }
+        const long vi = vec_index(J, q0 + b, x, c);
         if (vi < 0) continue;
-        const C conv = cscale(lds[x * bp + 3 * b + c], inv_m);
-        const C a = al[vi / 3];
-        if (MODE == XI_PLAIN) {
-            const C u = static_cast<const C*>(J.xin)[vi];
-            static_cast<C*>(J.yout)[vi] = cadd(conv, cmul(a, u));
-        } else if (MODE == XI_V) {
-            const KC* p = static_cast<const KC*>(J.p);
-            const KC* r0 = static_cast<const KC*>(J.r0);
-            const C y = cadd(conv, cmul(a, from_d<C>(to_d(p[vi]))));
-            const KC vk = from_d<KC>(to_d(y));
-            static_cast<KC*>(J.v)[vi] = vk;
-            const double2 d = cmulc(to_d(vk), to_d(r0[vi]));   // conj(r0) v
-            acc[0] += d.x;
-            acc[1] += d.y;
-        } else if (MODE == XI_S) {
-            const KC* t = static_cast<const KC*>(J.t);
-            const KC tk = t[vi];
-            const C y = cadd(conv, cmul(a, from_d<C>(to_d(tk))));
-            const KC sk = from_d<KC>(to_d(y));
-            static_cast<KC*>(J.s)[vi] = sk;
-            const double2 sd = to_d(sk), td = to_d(tk);
-            const double2 d = cmulc(td, sd);   // conj(s) t
-            acc[0] += d.x;
-            acc[1] += d.y;
-            acc[2] += sd.x * sd.x + sd.y * sd.y;
-        } else if (MODE == XI_RESID) {
-            const double2 y = cadd(to_d(conv), dcmul(to_d(a), J.P[vi]));
-            const double2 rr = csub(J.rhs[vi], y);
-            const KC rk = from_d<KC>(rr);
-            static_cast<KC*>(J.r)[vi] = rk;
-            const double2 d = cmulc(to_d(rk), to_d(static_cast<const KC*>(J.r0)[vi]));
-            acc[0] += d.x;
-            acc[1] += d.y;
-            acc[2] += rr.x * rr.x + rr.y * rr.y;
-        }
+        xi_element<C, KC, MODE>(J, vi, cscale(lds[x * bp + 3 * b + c], inv_m), al, acc);
     }
-    if (MODE == XI_PLAIN) return;
-    block_sum<3>(acc);
-    const int ntiles = Tile<Real>::tx(J);
-    if (!publish_partials<3>(J, jt.y, ntiles, acc)) return;
-    double sum[3];
-    sum_partials<3>(J, ntiles, sum);
-    if (threadIdx.x != 0) return;
-    KScalars* ks = J.ks;
-    const double2 dot = make_double2(sum[0], sum[1]);
-    if (MODE == XI_V) {
-        ks->alpha = ddiv(ks->rho, dot);                          // alpha = <r0,r>/<r0,Ap>
-    } else if (MODE == XI_S) {
-        ks->zeta = make_double2(dot.x / sum[2], dot.y / sum[2]);  // zeta = <At,t>/<At,At>
-    } else if (MODE == XI_RESID) {
-        ks->need_ru = 0;
-        ks->n_ru += 1;
-        ks->maxr2 = sum[2];
-        if (ks->status == JOB_ACTIVE) bicg_decide(ks, dot, sum[2]);
-    }
+    xi_finish<MODE>(J, jt.y, Tile<Real>::tx(J), acc);
 }
 
 // ----------------------------------------------------------------------------
@@ -379,33 +177,6 @@ k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const c
 // ----------------------------------------------------------------------------
 // Z pass: forward FFT, Green's-tensor multiply, inverse FFT, all in one kernel
 // ----------------------------------------------------------------------------
-// Green's spectrum symmetry: component (00,01,02,11,12,22) is odd in the axes
-// {}, {x,y}, {x,z}, {}, {y,z}, {} respectively, so only k <= L/2 per axis is
-// stored and Ghat(L-k) = +-Ghat(k).
-template <typename C>
-__device__ inline void green_apply(const C* __restrict__ G, const GridJob& J, int kx, int ky, int kz,
-                                   C& v0, C& v1, C& v2) {
-    int sx = 1, sy = 1, sz = 1;
-    if (2 * kx > J.lx) { kx = J.lx - kx; sx = -1; }
-    if (2 * ky > J.ly) { ky = J.ly - ky; sy = -1; }
-    if (2 * kz > J.lz) { kz = J.lz - kz; sz = -1; }
-    const long cs = (long)J.hz * J.hy * J.hx;
-    const long off = ((long)kz * J.hy + ky) * J.hx + kx;
-    const C g0 = G[off];
-    C g1 = G[cs + off];
-    C g2 = G[2 * cs + off];
-    const C g3 = G[3 * cs + off];
-    C g4 = G[4 * cs + off];
-    const C g5 = G[5 * cs + off];
-    if (sx * sy < 0) g1 = cscale(g1, -1);
-    if (sx * sz < 0) g2 = cscale(g2, -1);
-    if (sy * sz < 0) g4 = cscale(g4, -1);
-    const C a0 = v0, a1 = v1, a2 = v2;
-    v0 = cadd(cadd(cmul(g0, a0), cmul(g1, a1)), cmul(g2, a2));
-    v1 = cadd(cadd(cmul(g1, a0), cmul(g3, a1)), cmul(g4, a2));
-    v2 = cadd(cadd(cmul(g2, a0), cmul(g4, a1)), cmul(g5, a2));
-}
-
 // Last forward stage (radix R), multiply in registers, first inverse stage
 // (same radix, so every thread already holds its inputs).
 template <int R, int CAP, typename C, typename TW, typename LoadF, typename StoreF, typename MulF>
@@ -624,81 +395,109 @@ k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm
 // host launchers
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal>
-static void matvec_impl(const LaunchSet& ls, const cx_t<Real>* tw, int xf, int xi, int gate, hipStream_t st) {
-    const bool c64 = sizeof(Real) == 4;
-    const int2* mx = c64 ? ls.map_x32 : ls.map_x64;
-    const int2* my = c64 ? ls.map_y32 : ls.map_y64;
-    const int2* mz = c64 ? ls.map_z32 : ls.map_z64;
-    const int nx = c64 ? ls.n_x32 : ls.n_x64;
-    const int ny = c64 ? ls.n_y32 : ls.n_y64;
-    const int nz = c64 ? ls.n_z32 : ls.n_z64;
-    const size_t lx = c64 ? ls.lds_x32 : ls.lds_x64;
-    const size_t ly = c64 ? ls.lds_y32 : ls.lds_y64;
-    const size_t lz = c64 ? ls.lds_z32 : ls.lds_z64;
-    if (nx == 0) return;
-    const dim3 blk(FFT_THREADS);
+static void xfwd_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xf, int gate,
+                         hipStream_t st) {
+    if (pm.n == 0) return;
+    const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xf) {
-        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
         default: throw Error(ERR_INVALID, "bad X-forward mode");
     }
-    hipLaunchKernelGGL((k_ypass<Real, -1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, gate);
-    hipLaunchKernelGGL((k_zpass<Real>), dim3(nz), blk, lz, st, ls.jobs, mz, tw, gate);
-    hipLaunchKernelGGL((k_ypass<Real, 1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, gate);
+}
+
+template <typename Real, typename KReal>
+static void xinv_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xi, int gate,
+                         hipStream_t st) {
+    if (pm.n == 0) return;
+    const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xi) {
-        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
-        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, gate); break;
+        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
         default: throw Error(ERR_INVALID, "bad X-inverse mode");
     }
+}
+
+template <typename Real>
+static void ypass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int dir, int gate,
+                          hipStream_t st) {
+    if (pm.n == 0) return;
+    const dim3 grid(pm.n), blk(FFT_THREADS);
+    if (dir < 0) hipLaunchKernelGGL((k_ypass<Real, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate);
+    else hipLaunchKernelGGL((k_ypass<Real, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate);
+}
+
+template <typename Real>
+static void zpass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int gate, hipStream_t st) {
+    if (pm.n == 0) return;
+    hipLaunchKernelGGL((k_zpass<Real>), dim3(pm.n), dim3(FFT_THREADS), pm.lds, st, jobs, pm.map, tw, gate);
+}
+
+// One pass of the matvec (0 X fwd, 1 Y fwd, 2 Z, 3 Y inv, 4 X inv): the
+// generic launch and the specialised launch of the batch.
+template <typename Real, typename KReal>
+static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
+    constexpr bool c64 = sizeof(Real) == 4;
+    constexpr bool k32 = sizeof(KReal) == 4;
+    const int pr = c64 ? 0 : 1;
+    const cx_t<Real>* t = c64 ? reinterpret_cast<const cx_t<Real>*>(tw.tw32) : reinterpret_cast<const cx_t<Real>*>(tw.tw64);
+    switch (k) {
+        case 0:
+            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, st);
+            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, st);
+            break;
+        case 1:
+        case 3:
+            ypass_generic<Real>(ls.pm[pr][PASS_Y][0], ls.jobs, t, k == 1 ? -1 : 1, gate, st);
+            launch_fast_y(ls.pm[pr][PASS_Y][1], ls.jobs, tw, c64, k == 1 ? -1 : 1, gate, st);
+            break;
+        case 2:
+            zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, st);
+            launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, st);
+            break;
+        default:
+            xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
+            launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, st);
+            break;
+    }
+}
+
+template <typename Real, typename KReal>
+static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
+    const int pr = sizeof(Real) == 4 ? 0 : 1;
+    if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
+    for (int k = 0; k < 5; ++k) pass_impl<Real, KReal>(k, ls, tw, xf, xi, gate, st);
     TDDA_HIP(hipGetLastError());
 }
 
 void launch_matvec(const LaunchSet& ls, const TwiddlePools& tw, bool c64, int xf_mode, int xi_mode, int gate,
                    hipStream_t st) {
     if (c64) {
-        matvec_impl<float, float>(ls, tw.tw32, xf_mode, xi_mode, gate, st);
+        matvec_impl<float, float>(ls, tw, xf_mode, xi_mode, gate, st);
     } else if (ls.refined) {
         // FP64 transform feeding c64 Krylov vectors: reliable updates only
         if (!(xf_mode == XF_SOL && xi_mode == XI_RESID) && !(xf_mode == XF_PLAIN && xi_mode == XI_PLAIN))
             throw Error(ERR_INVALID, "FP64 transform in refined mode supports residual/plain only");
-        matvec_impl<double, float>(ls, tw.tw64, xf_mode, xi_mode, gate, st);
+        matvec_impl<double, float>(ls, tw, xf_mode, xi_mode, gate, st);
     } else {
-        matvec_impl<double, double>(ls, tw.tw64, xf_mode, xi_mode, gate, st);
+        matvec_impl<double, double>(ls, tw, xf_mode, xi_mode, gate, st);
     }
 }
 
 template <typename Real, typename KReal>
-static void matvec_timed_impl(const LaunchSet& ls, const cx_t<Real>* tw, hipStream_t st, int reps, double* ms5) {
-    const bool c64 = sizeof(Real) == 4;
-    const int2* mx = c64 ? ls.map_x32 : ls.map_x64;
-    const int2* my = c64 ? ls.map_y32 : ls.map_y64;
-    const int2* mz = c64 ? ls.map_z32 : ls.map_z64;
-    const int nx = c64 ? ls.n_x32 : ls.n_x64;
-    const int ny = c64 ? ls.n_y32 : ls.n_y64;
-    const int nz = c64 ? ls.n_z32 : ls.n_z64;
-    const size_t lx = c64 ? ls.lds_x32 : ls.lds_x64;
-    const size_t ly = c64 ? ls.lds_y32 : ls.lds_y64;
-    const size_t lz = c64 ? ls.lds_z32 : ls.lds_z64;
+static void matvec_timed_impl(const LaunchSet& ls, const TwiddlePools& tw, hipStream_t st, int reps, double* ms5) {
     hipEvent_t ev[6];
     for (auto& e : ev) TDDA_HIP(hipEventCreate(&e));
-    const dim3 blk(FFT_THREADS);
-    const int g = GATE_ALWAYS;
     for (int r = 0; r < reps; ++r) {
         TDDA_HIP(hipEventRecord(ev[0], st));
-        hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, g);
-        TDDA_HIP(hipEventRecord(ev[1], st));
-        hipLaunchKernelGGL((k_ypass<Real, -1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, g);
-        TDDA_HIP(hipEventRecord(ev[2], st));
-        hipLaunchKernelGGL((k_zpass<Real>), dim3(nz), blk, lz, st, ls.jobs, mz, tw, g);
-        TDDA_HIP(hipEventRecord(ev[3], st));
-        hipLaunchKernelGGL((k_ypass<Real, 1>), dim3(ny), blk, ly, st, ls.jobs, my, tw, g);
-        TDDA_HIP(hipEventRecord(ev[4], st));
-        hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), dim3(nx), blk, lx, st, ls.jobs, mx, tw, g);
-        TDDA_HIP(hipEventRecord(ev[5], st));
+        for (int k = 0; k < 5; ++k) {
+            pass_impl<Real, KReal>(k, ls, tw, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+            TDDA_HIP(hipEventRecord(ev[k + 1], st));
+        }
         TDDA_HIP(hipEventSynchronize(ev[5]));
         for (int k = 0; k < 5; ++k) {
             float ms = 0.f;
@@ -713,11 +512,11 @@ static void matvec_timed_impl(const LaunchSet& ls, const cx_t<Real>* tw, hipStre
 void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, hipStream_t st, int reps,
                          double* ms5) {
     if (c64)
-        matvec_timed_impl<float, float>(ls, tw.tw32, st, reps, ms5);
+        matvec_timed_impl<float, float>(ls, tw, st, reps, ms5);
     else if (ls.refined)
-        matvec_timed_impl<double, float>(ls, tw.tw64, st, reps, ms5);
+        matvec_timed_impl<double, float>(ls, tw, st, reps, ms5);
     else
-        matvec_timed_impl<double, double>(ls, tw.tw64, st, reps, ms5);
+        matvec_timed_impl<double, double>(ls, tw, st, reps, ms5);
 }
 
 void launch_update(const LaunchSet& ls, int gate, hipStream_t st) {
@@ -741,14 +540,36 @@ void launch_init(const LaunchSet& ls, bool warm, hipStream_t st) {
 // ----------------------------------------------------------------------------
 // tile geometry
 // ----------------------------------------------------------------------------
+static unsigned long long quotient_magic(int d) {
+    return ((1ull << 32) + (unsigned long long)d - 1ull) / (unsigned long long)d;
+}
+
 void plan_tiles(GridJob& J) {
     auto cdiv = [](long a, long b) { return (int)((a + b - 1) / b); };
-    J.bx32 = std::max(1, Caps<float>::X / (3 * J.lx));
-    J.by32 = std::max(1, Caps<float>::Y / J.ly);
-    J.bz32 = std::max(1, Caps<float>::Z / J.lz);
-    J.bx64 = std::max(1, Caps<double>::X / (3 * J.lx));
-    J.by64 = std::max(1, Caps<double>::Y / J.ly);
-    J.bz64 = std::max(1, Caps<double>::Z / J.lz);
+    size_t unused = 0;
+    J.mlx = quotient_magic(J.lx);
+    J.mny = quotient_magic(J.ny);
+    J.fx = fast_tiles(J.lx, PASS_X, true, &J.bx32, &unused) ? 1 : 0;
+    if (J.fx) {
+        fast_tiles(J.lx, PASS_X, false, &J.bx64, &unused);
+    } else {
+        J.bx32 = std::max(1, Caps<float>::X / (3 * J.lx));
+        J.bx64 = std::max(1, Caps<double>::X / (3 * J.lx));
+    }
+    J.fy = fast_tiles(J.ly, PASS_Y, true, &J.by32, &unused) ? 1 : 0;
+    if (J.fy) {
+        fast_tiles(J.ly, PASS_Y, false, &J.by64, &unused);
+    } else {
+        J.by32 = std::max(1, Caps<float>::Y / J.ly);
+        J.by64 = std::max(1, Caps<double>::Y / J.ly);
+    }
+    J.fz = fast_tiles(J.lz, PASS_Z, true, &J.bz32, &unused) ? 1 : 0;
+    if (J.fz) {
+        fast_tiles(J.lz, PASS_Z, false, &J.bz64, &unused);
+    } else {
+        J.bz32 = std::max(1, Caps<float>::Z / J.lz);
+        J.bz64 = std::max(1, Caps<double>::Z / J.lz);
+    }
     const long nrows = (long)J.ny * J.nz;
     J.tx32 = cdiv(nrows, J.bx32);
     J.ty32 = cdiv(3L * J.nz * J.lx, J.by32);
@@ -759,17 +580,19 @@ void plan_tiles(GridJob& J) {
     J.tupd = cdiv(3L * J.ndip, (long)FFT_THREADS * UPD_EPT);
 }
 
-size_t lds_bytes_x(const GridJob& J, bool c64) {
-    const int b = c64 ? J.bx32 : J.bx64;
-    return (size_t)J.lx * ((3 * b) | 1) * (c64 ? sizeof(float2) : sizeof(double2));
-}
-size_t lds_bytes_y(const GridJob& J, bool c64) {
-    const int b = c64 ? J.by32 : J.by64;
-    return (size_t)J.ly * (b | 1) * (c64 ? sizeof(float2) : sizeof(double2));
-}
-size_t lds_bytes_z(const GridJob& J, bool c64) {
-    const int b = c64 ? J.bz32 : J.bz64;
-    return (size_t)3 * J.lz * (b | 1) * (c64 ? sizeof(float2) : sizeof(double2));
+size_t lds_bytes(const GridJob& J, int pass, bool c64) {
+    const int L = pass == PASS_X ? J.lx : pass == PASS_Y ? J.ly : J.lz;
+    const bool fast = pass == PASS_X ? J.fx : pass == PASS_Y ? J.fy : J.fz;
+    const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
+    if (fast) {
+        int per = 0;
+        size_t bytes = 0;
+        fast_tiles(L, pass, c64, &per, &bytes);
+        return bytes;
+    }
+    if (pass == PASS_X) return (size_t)J.lx * ((3 * (c64 ? J.bx32 : J.bx64)) | 1) * cs;
+    if (pass == PASS_Y) return (size_t)J.ly * ((c64 ? J.by32 : J.by64) | 1) * cs;
+    return (size_t)3 * J.lz * ((c64 ? J.bz32 : J.bz64) | 1) * cs;
 }
 
 }  // namespace tdda

@@ -429,11 +429,51 @@ void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool re
     build(systems, ks, refined, st, true);
 }
 
+namespace {
+
+// Z tiles of one job ordered so that tiles whose pencils are x/y mirrors of
+// each other (they read the same entries of the octant Green's spectrum) are
+// consecutive: sort by the mirrored centre pencil of the tile.
+std::vector<int> z_tile_order(const GridJob& J, int per_tile, int ntiles) {
+    std::vector<std::pair<long, int>> key(ntiles);
+    for (int t = 0; t < ntiles; ++t) {
+        const long pc = (long)t * per_tile + per_tile / 2;
+        const int ky = (int)(pc / J.lx), kx = (int)(pc % J.lx);
+        const int my = std::min(ky, J.ly - ky), mx = std::min(kx, J.lx - kx);
+        key[t] = {((long)my * J.lx + mx), t};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<long, int>& a, const std::pair<long, int>& b) { return a.first < b.first; });
+    std::vector<int> order(ntiles);
+    for (int t = 0; t < ntiles; ++t) order[t] = key[t].second;
+    return order;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Give XCD x a contiguous range of the logical order so that tiles
+// adjacent in that order run together on one XCD and share its L2.
+void xcd_permute(std::vector<int2>& m) {
+    constexpr size_t kXcd = 8;
+    const size_t n = m.size();
+    if (n < 2 * kXcd) return;
+    std::vector<int2> out(n);
+    size_t start = 0;
+    for (size_t x = 0; x < kXcd; ++x) {
+        const size_t cnt = (n - x + kXcd - 1) / kXcd;
+        for (size_t r = 0; r < cnt; ++r) out[x + kXcd * r] = m[start + r];
+        start += cnt;
+    }
+    m.swap(out);
+}
+
+}  // namespace
+
 void Batch::build(const std::vector<System*>& systems, const std::vector<KScalars*>& ks, bool refined,
                   hipStream_t st, bool interleave_shared) {
     hjobs.clear();
-    std::vector<int2> m[7];
-    size_t lds[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<int2> m[2][3][2];
+    std::vector<int2> mu;
+    size_t lds[2][3][2] = {};
     for (size_t j = 0; j < systems.size(); ++j) {
         System* s = systems[j];
         TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
@@ -441,53 +481,62 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         J.ks = j < ks.size() ? ks[j] : nullptr;
         hjobs.push_back(J);
         const int jj = (int)j;
-        for (int t = 0; t < J.tx32; ++t) m[0].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.ty32; ++t) m[1].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.tx64; ++t) m[3].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.ty64; ++t) m[4].push_back(make_int2(jj, t));
-        for (int t = 0; t < J.tupd; ++t) m[6].push_back(make_int2(jj, t));
-        lds[0] = std::max(lds[0], lds_bytes_x(J, true));
-        lds[1] = std::max(lds[1], lds_bytes_y(J, true));
-        lds[2] = std::max(lds[2], lds_bytes_z(J, true));
-        lds[3] = std::max(lds[3], lds_bytes_x(J, false));
-        lds[4] = std::max(lds[4], lds_bytes_y(J, false));
-        lds[5] = std::max(lds[5], lds_bytes_z(J, false));
+        for (int pr = 0; pr < 2; ++pr) {
+            const bool c64 = pr == 0;
+            const int tx = c64 ? J.tx32 : J.tx64, ty = c64 ? J.ty32 : J.ty64;
+            for (int t = 0; t < tx; ++t) m[pr][PASS_X][J.fx].push_back(make_int2(jj, t));
+            for (int t = 0; t < ty; ++t) m[pr][PASS_Y][J.fy].push_back(make_int2(jj, t));
+            size_t& lx = lds[pr][PASS_X][J.fx];
+            size_t& ly = lds[pr][PASS_Y][J.fy];
+            size_t& lz = lds[pr][PASS_Z][J.fz];
+            lx = std::max(lx, lds_bytes(J, PASS_X, c64));
+            ly = std::max(ly, lds_bytes(J, PASS_Y, c64));
+            lz = std::max(lz, lds_bytes(J, PASS_Z, c64));
+        }
+        for (int t = 0; t < J.tupd; ++t) mu.push_back(make_int2(jj, t));
     }
-    // Z-pass maps: group systems by operator
+    // Z maps: mirror-grouped tile order per job; systems that share an
+    // operator have their tiles interleaved in chunks of 8
     std::vector<std::vector<int>> groups;
     std::map<const Operator*, size_t> gidx;
     for (size_t j = 0; j < systems.size(); ++j) {
-        const Operator* o = interleave_shared ? systems[j]->op.get() : nullptr;
         if (!interleave_shared) { groups.push_back({(int)j}); continue; }
+        const Operator* o = systems[j]->op.get();
         auto it = gidx.find(o);
         if (it == gidx.end()) { gidx[o] = groups.size(); groups.push_back({(int)j}); }
         else groups[it->second].push_back((int)j);
     }
-    for (int pass = 0; pass < 2; ++pass) {
-        std::vector<int2>& mz = pass == 0 ? m[2] : m[5];
+    for (int pr = 0; pr < 2; ++pr) {
         for (const auto& g : groups) {
-            const int T = pass == 0 ? hjobs[g[0]].tz32 : hjobs[g[0]].tz64;
-            if (g.size() == 1) {
-                for (int t = 0; t < T; ++t) mz.push_back(make_int2(g[0], t));
-                continue;
-            }
+            const GridJob& J0 = hjobs[g[0]];
+            const int T = pr == 0 ? J0.tz32 : J0.tz64;
+            const int per = pr == 0 ? J0.bz32 : J0.bz64;
+            const std::vector<int> order = z_tile_order(J0, per, T);
+            std::vector<int2>& mz = m[pr][PASS_Z][J0.fz];
             for (int t0 = 0; t0 < T; t0 += 8) {
                 const int t1 = std::min(T, t0 + 8);
                 for (int j : g)
-                    for (int t = t0; t < t1; ++t) mz.push_back(make_int2(j, t));
+                    for (int t = t0; t < t1; ++t) mz.push_back(make_int2(j, order[t]));
             }
         }
+        for (int k = 0; k < 2; ++k) xcd_permute(m[pr][PASS_Z][k]);
     }
-    size_t total = 0;
-    for (auto& v : m) total += v.size();
+    size_t total = mu.size();
+    for (auto& a : m)
+        for (auto& b : a)
+            for (auto& v : b) total += v.size();
     std::vector<int2>& all = hmaps_;   // kept alive: the upload is asynchronous
     all.clear();
     all.reserve(total);
-    size_t off[7];
-    for (int i = 0; i < 7; ++i) {
-        off[i] = all.size();
-        all.insert(all.end(), m[i].begin(), m[i].end());
-    }
+    size_t off[2][3][2];
+    for (int pr = 0; pr < 2; ++pr)
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) {
+                off[pr][p][k] = all.size();
+                all.insert(all.end(), m[pr][p][k].begin(), m[pr][p][k].end());
+            }
+    const size_t off_u = all.size();
+    all.insert(all.end(), mu.begin(), mu.end());
     if (maps_.size() < total) maps_.alloc(std::max<size_t>(total, 1));
     if (total) maps_.upload(all.data(), total, st);
     if (djobs_.size() < hjobs.size()) djobs_.alloc(std::max<size_t>(hjobs.size(), 1));
@@ -495,15 +544,16 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     const int2* base = maps_.get();
     ls = LaunchSet{};
     ls.jobs = djobs_.get();
-    ls.map_x32 = base + off[0]; ls.n_x32 = (int)m[0].size();
-    ls.map_y32 = base + off[1]; ls.n_y32 = (int)m[1].size();
-    ls.map_z32 = base + off[2]; ls.n_z32 = (int)m[2].size();
-    ls.map_x64 = base + off[3]; ls.n_x64 = (int)m[3].size();
-    ls.map_y64 = base + off[4]; ls.n_y64 = (int)m[4].size();
-    ls.map_z64 = base + off[5]; ls.n_z64 = (int)m[5].size();
-    ls.map_upd = base + off[6]; ls.n_upd = (int)m[6].size();
-    ls.lds_x32 = lds[0]; ls.lds_y32 = lds[1]; ls.lds_z32 = lds[2];
-    ls.lds_x64 = lds[3]; ls.lds_y64 = lds[4]; ls.lds_z64 = lds[5];
+    for (int pr = 0; pr < 2; ++pr)
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) {
+                PassMap& pm = ls.pm[pr][p][k];
+                pm.map = base + off[pr][p][k];
+                pm.n = (int)m[pr][p][k].size();
+                pm.lds = lds[pr][p][k];
+            }
+    ls.map_upd = base + off_u;
+    ls.n_upd = (int)mu.size();
     ls.refined = refined;
 }
 
