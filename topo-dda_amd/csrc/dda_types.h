@@ -34,8 +34,11 @@ struct KScalars {
     int refined;        // 1: FP32 Krylov with FP64 reliable updates
     int iterations;     // reported iteration count (reference ITERATION)
     int n_ru;           // number of reliable updates done
-    int pad;
+    int pending;        // 1: P += alpha p + zeta t, r = t - zeta s of the last step not yet applied
 };
+
+// doubles per tile in the per-job partial-sum buffer
+constexpr int PARTIAL_STRIDE = 8;
 
 // One system (diag(al) + A) P = rhs on an Nx x Ny x Nz box.
 struct GridJob {
@@ -52,7 +55,7 @@ struct GridJob {
     unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients
     const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
     // operator data
-    const float2* green32;     // [6][hz][hy][hx]
+    const float2* green32;     // [hz][hy][hx][6] (octant spectrum, point-major)
     const double2* green64;
     const float2* al32;        // [ndip] 1/alpha
     const double2* al64;
@@ -66,7 +69,7 @@ struct GridJob {
     const void* xin;
     void* yout;
     KScalars* ks;
-    double* partial;           // [max tiles][6]
+    double* partial;           // [max tiles][PARTIAL_STRIDE]
     unsigned* counter;
 };
 

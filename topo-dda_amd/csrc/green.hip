@@ -111,9 +111,15 @@ __global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ i
     out[(((long)c * outer + a) * h + k) * inner + i] = acc;
 }
 
-__global__ void k_to_c64(const double2* __restrict__ in, float2* __restrict__ out, long n) {
+__global__ void k_green_interleave(const double2* __restrict__ in, long npts, double2* __restrict__ out64,
+                                   float2* __restrict__ out32) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+    if (i >= 6 * npts) return;
+    const long p = i / 6;
+    const int c = (int)(i - 6 * p);
+    const double2 v = in[c * npts + p];
+    out64[i] = v;
+    out32[i] = make_float2((float)v.x, (float)v.y);
 }
 
 void launch_green_octant(const GreenParams& P, const double* si, const double* ci, double2* out, hipStream_t st) {
@@ -133,8 +139,9 @@ void launch_green_dft(const double2* in, double2* out, int outer, int no, int h,
     TDDA_HIP(hipGetLastError());
 }
 
-void launch_to_c64(const double2* in, float2* out, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_to_c64, dim3((n + 255) / 256), dim3(256), 0, st, in, out, n);
+void launch_green_interleave(const double2* in, long npts, double2* out64, float2* out32, hipStream_t st) {
+    const long n = 6 * npts;
+    hipLaunchKernelGGL(k_green_interleave, dim3((n + 255) / 256), dim3(256), 0, st, in, npts, out64, out32);
     TDDA_HIP(hipGetLastError());
 }
 

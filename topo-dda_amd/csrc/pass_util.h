@@ -34,21 +34,30 @@ __device__ inline bool gate_open(const GridJob& J, int gate) {
 }
 
 // Deterministic block reduction of NV doubles; every thread gets the sums.
+// Butterfly over the 64 lanes of each wave (a fixed pattern, so the result
+// is reproducible), then the FFT_THREADS/64 wave sums in wave order.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV]) {
-    __shared__ double red[NV][FFT_THREADS];
+    constexpr int NW = FFT_THREADS / 64;
+    __shared__ double red[NV][NW];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (int s = FFT_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
+    for (int k = 0; k < NV; ++k) {
 #pragma unroll
-            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        }
-        __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
     }
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = red[k][0];
+        for (int k = 0; k < NV; ++k) red[k][threadIdx.x >> 6] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = red[k][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += red[k][w];
+        v[k] = s;
+    }
+    __syncthreads();   // red may be reused by the next call
 }
 
 // Publish this block's partial sums; returns true (in every thread) for the
@@ -59,7 +68,7 @@ __device__ inline bool publish_partials(const GridJob& J, int tile, int ntiles, 
     __shared__ int s_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) J.partial[(long)tile * 6 + k] = v[k];
+        for (int k = 0; k < NV; ++k) J.partial[(long)tile * PARTIAL_STRIDE + k] = v[k];
         __threadfence();
         const unsigned prev = atomicAdd(J.counter, 1u);
         s_last = (prev + 1 == (unsigned)ntiles) ? 1 : 0;
@@ -78,7 +87,7 @@ __device__ inline void sum_partials(const GridJob& J, int ntiles, double (&out)[
     for (int t = threadIdx.x; t < ntiles; t += FFT_THREADS) {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-            out[k] += __hip_atomic_load(&J.partial[(long)t * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[k] += __hip_atomic_load(&J.partial[(long)t * PARTIAL_STRIDE + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     block_sum<NV>(out);
     if (threadIdx.x == 0) __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -117,19 +126,62 @@ __device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2) {
     ks->iter = next;
 }
 
+// One BiCGSTAB step's end (DDAModel.cpp:981-1004) in refined mode first:
+// a residual small against the last reliable update, or one that meets the
+// tolerance or the iteration limit, asks for the true residual instead.
+__device__ inline void step_decide(KScalars* ks, double2 rho_new, double rn2) {
+    if (ks->refined) {
+        const double rel2 = rn2 / ks->enorm2;
+        const bool trigger = rn2 <= ks->ru_delta * ks->ru_delta * ks->maxr2 ||
+                             rel2 <= ks->tol * ks->tol || ks->iter + 1 >= ks->maxit;
+        if (trigger && ks->iter >= 0) {
+            ks->need_ru = 1;     // true residual decides (k_xinv XI_RESID)
+            ks->rnorm2 = rn2;
+            return;
+        }
+        if (rn2 > ks->maxr2) ks->maxr2 = rn2;
+    }
+    bicg_decide(ks, rho_new, rn2);
+}
+
 // ----------------------------------------------------------------------------
 // X forward input: the vector fed to the transform, with the BiCGSTAB vector
-// update fused in (DDAModel.cpp:976, 979).  vi < 0: outside the geometry.
+// operations fused in.  The end-of-step update of the previous step
+// (DDAModel.cpp:981-986: P += alpha p + zeta t, which equals the reference's
+// alpha p + zeta r - alpha zeta v since t = r - alpha v; r = t - zeta s) is
+// applied here, element by element, when ks->pending is set; then
+// p = r + beta (p - zeta v) (DDAModel.cpp:976) or t = r - alpha v (:979).
+// vi < 0: outside the geometry.
 // ----------------------------------------------------------------------------
 struct XfScalars {
     double2 alpha, beta, zeta;
+    int pending;
 };
 template <int MODE>
 __device__ inline XfScalars xf_scalars(const GridJob& J) {
-    XfScalars s{make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    if (MODE == XF_P) { s.beta = J.ks->beta; s.zeta = J.ks->zeta; }
-    if (MODE == XF_T) { s.alpha = J.ks->alpha; }
+    XfScalars s{make_double2(0, 0), make_double2(0, 0), make_double2(0, 0), 0};
+    const KScalars* ks = J.ks;
+    if (MODE == XF_P || MODE == XF_SOL) {
+        s.alpha = ks->alpha;
+        s.beta = ks->beta;
+        s.zeta = ks->zeta;
+        s.pending = ks->pending;
+    }
+    if (MODE == XF_T) s.alpha = ks->alpha;
     return s;
+}
+
+// the pending step update of one element; returns the new P
+template <typename KC>
+__device__ inline double2 apply_pending(const GridJob& J, long vi, const XfScalars& sc, double2 pv, bool write_r) {
+    const double2 td = to_d(static_cast<const KC*>(J.t)[vi]);
+    const double2 Pn = cadd(J.P[vi], cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
+    J.P[vi] = Pn;
+    if (write_r) {
+        const double2 sd = to_d(static_cast<const KC*>(J.s)[vi]);
+        static_cast<KC*>(J.r)[vi] = from_d<KC>(csub(td, dcmul(sc.zeta, sd)));
+    }
+    return Pn;
 }
 
 template <typename C, typename KC, int MODE>
@@ -138,18 +190,34 @@ __device__ inline C xf_element(const GridJob& J, long vi, const XfScalars& sc) {
     if (MODE == XF_PLAIN) {
         return static_cast<const C*>(J.xin)[vi];
     } else if (MODE == XF_SOL) {
+        if (sc.pending) {
+            const double2 pv = to_d(static_cast<const KC*>(J.p)[vi]);
+            return from_d<C>(apply_pending<KC>(J, vi, sc, pv, false));
+        }
         return from_d<C>(J.P[vi]);
     } else if (MODE == XF_P) {
         KC* r = static_cast<KC*>(J.r);
         KC* p = static_cast<KC*>(J.p);
-        KC* v = static_cast<KC*>(J.v);
-        const double2 w = csub(to_d(p[vi]), dcmul(sc.zeta, to_d(v[vi])));   // p - u, u = zeta v
-        const KC pk = from_d<KC>(cadd(to_d(r[vi]), dcmul(sc.beta, w)));
+        const KC* v = static_cast<const KC*>(J.v);
+        const double2 pv = to_d(p[vi]);
+        double2 rv;
+        if (sc.pending) {
+            const double2 td = to_d(static_cast<const KC*>(J.t)[vi]);
+            const double2 sd = to_d(static_cast<const KC*>(J.s)[vi]);
+            J.P[vi] = cadd(J.P[vi], cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
+            const KC rk = from_d<KC>(csub(td, dcmul(sc.zeta, sd)));
+            r[vi] = rk;
+            rv = to_d(rk);
+        } else {
+            rv = to_d(r[vi]);
+        }
+        const double2 w = csub(pv, dcmul(sc.zeta, to_d(v[vi])));   // p - u, u = zeta v
+        const KC pk = from_d<KC>(cadd(rv, dcmul(sc.beta, w)));
         p[vi] = pk;
         return from_d<C>(to_d(pk));
     } else {   // XF_T
-        KC* r = static_cast<KC*>(J.r);
-        KC* v = static_cast<KC*>(J.v);
+        const KC* r = static_cast<const KC*>(J.r);
+        const KC* v = static_cast<const KC*>(J.v);
         KC* t = static_cast<KC*>(J.t);
         const KC tk = from_d<KC>(csub(to_d(r[vi]), dcmul(sc.alpha, to_d(v[vi]))));
         t[vi] = tk;
@@ -166,10 +234,16 @@ __device__ inline long vec_index(const GridJob& J, int q, int x, int c) {
 
 // ----------------------------------------------------------------------------
 // X inverse epilogue: y = conv + al u and what the mode does with it.
-// conv is already divided by M.  acc gathers the per-thread dot partials.
+// conv is already divided by M.  acc gathers the per-thread dot partials:
+//   XI_V     : <r0, v>
+//   XI_S     : <s, t>, <s, s>, <r0, t>, <r0, s>, <t, t>
+//   XI_RESID : <r0, r>, <r, r>
 // ----------------------------------------------------------------------------
+constexpr int XI_NACC = 8;
+
 template <typename C, typename KC, int MODE>
-__device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __restrict__ al, double (&acc)[3]) {
+__device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __restrict__ al,
+                                  double (&acc)[XI_NACC]) {
     const C a = al[vi / 3];
     if (MODE == XI_PLAIN) {
         const C u = static_cast<const C*>(J.xin)[vi];
@@ -190,10 +264,18 @@ __device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __
         const KC sk = from_d<KC>(to_d(y));
         static_cast<KC*>(J.s)[vi] = sk;
         const double2 sd = to_d(sk), td = to_d(tk);
-        const double2 d = cmulc(td, sd);   // conj(s) t
+        const double2 rz = to_d(static_cast<const KC*>(J.r0)[vi]);
+        const double2 d = cmulc(td, sd);    // conj(s) t
+        const double2 e = cmulc(td, rz);    // conj(r0) t
+        const double2 f = cmulc(sd, rz);    // conj(r0) s
         acc[0] += d.x;
         acc[1] += d.y;
         acc[2] += sd.x * sd.x + sd.y * sd.y;
+        acc[3] += e.x;
+        acc[4] += e.y;
+        acc[5] += f.x;
+        acc[6] += f.y;
+        acc[7] += td.x * td.x + td.y * td.y;
     } else if (MODE == XI_RESID) {
         const double2 y = cadd(to_d(conv), dcmul(to_d(a), J.P[vi]));
         const double2 rr = csub(J.rhs[vi], y);
@@ -206,23 +288,41 @@ __device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __
     }
 }
 
-// Reduce the dot partials of a job over its X tiles and set alpha / zeta /
-// the reliable-update state in the completing block (all threads call it).
+// Reduce the dot partials of a job over its X tiles and advance its scalars
+// in the completing block (all threads call it).
+//   XI_V     : alpha = <r0,r>/<r0,Ap>; the pending update was applied by the
+//              X forward pass of this matvec
+//   XI_S     : zeta = <At,t>/<At,At>; with r = t - zeta s,
+//              <r0, r> = <r0,t> - zeta <r0,s> and
+//              |r|^2 = |t|^2 - |<s,t>|^2/|s|^2 (zeta minimises |t - zeta s|);
+//              the vector update becomes pending and the step is decided
+//   XI_RESID : true residual of the updated P (the X forward pass applied the
+//              pending update): reliable-update bookkeeping and decision
 template <int MODE>
-__device__ inline void xi_finish(const GridJob& J, int tile, int ntiles, double (&acc)[3]) {
+__device__ inline void xi_finish(const GridJob& J, int tile, int ntiles, double (&acc)[XI_NACC]) {
     if (MODE == XI_PLAIN) return;
-    block_sum<3>(acc);
-    if (!publish_partials<3>(J, tile, ntiles, acc)) return;
-    double sum[3];
-    sum_partials<3>(J, ntiles, sum);
+    block_sum<XI_NACC>(acc);
+    if (!publish_partials<XI_NACC>(J, tile, ntiles, acc)) return;
+    double sum[XI_NACC];
+    sum_partials<XI_NACC>(J, ntiles, sum);
     if (threadIdx.x != 0) return;
     KScalars* ks = J.ks;
     const double2 dot = make_double2(sum[0], sum[1]);
     if (MODE == XI_V) {
-        ks->alpha = ddiv(ks->rho, dot);                          // alpha = <r0,r>/<r0,Ap>
+        ks->alpha = ddiv(ks->rho, dot);
+        ks->pending = 0;
     } else if (MODE == XI_S) {
-        ks->zeta = make_double2(dot.x / sum[2], dot.y / sum[2]);  // zeta = <At,t>/<At,At>
+        const double ss = sum[2];
+        const double2 zeta = make_double2(dot.x / ss, dot.y / ss);
+        ks->zeta = zeta;
+        const double2 r0t = make_double2(sum[3], sum[4]);
+        const double2 r0s = make_double2(sum[5], sum[6]);
+        const double2 rho_new = csub(r0t, dcmul(zeta, r0s));
+        const double rn2 = fmax(sum[7] - (dot.x * dot.x + dot.y * dot.y) / ss, 0.0);
+        ks->pending = 1;
+        step_decide(ks, rho_new, rn2);
     } else if (MODE == XI_RESID) {
+        ks->pending = 0;
         ks->need_ru = 0;
         ks->n_ru += 1;
         ks->maxr2 = sum[2];
@@ -236,6 +336,19 @@ __device__ inline void xi_finish(const GridJob& J, int tile, int ntiles, double 
 // {y,z}, {} respectively, so only k <= L/2 per axis is stored and
 // Ghat(L-k) = +-Ghat(k).
 // ----------------------------------------------------------------------------
+// The six components of one spectral point are one contiguous record.
+__device__ inline void green_load6(const float2* __restrict__ G, long off, float2 (&g)[6]) {
+    const float4* q = reinterpret_cast<const float4*>(G + 6 * off);
+    const float4 a = q[0], b = q[1], c = q[2];
+    g[0] = make_float2(a.x, a.y); g[1] = make_float2(a.z, a.w);
+    g[2] = make_float2(b.x, b.y); g[3] = make_float2(b.z, b.w);
+    g[4] = make_float2(c.x, c.y); g[5] = make_float2(c.z, c.w);
+}
+__device__ inline void green_load6(const double2* __restrict__ G, long off, double2 (&g)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = G[6 * off + k];
+}
+
 template <typename C>
 __device__ inline void green_apply(const C* __restrict__ G, const GridJob& J, int kx, int ky, int kz,
                                    C& v0, C& v1, C& v2) {
@@ -243,21 +356,16 @@ __device__ inline void green_apply(const C* __restrict__ G, const GridJob& J, in
     if (2 * kx > J.lx) { kx = J.lx - kx; sx = -1; }
     if (2 * ky > J.ly) { ky = J.ly - ky; sy = -1; }
     if (2 * kz > J.lz) { kz = J.lz - kz; sz = -1; }
-    const long cs = (long)J.hz * J.hy * J.hx;
     const long off = ((long)kz * J.hy + ky) * J.hx + kx;
-    const C g0 = G[off];
-    C g1 = G[cs + off];
-    C g2 = G[2 * cs + off];
-    const C g3 = G[3 * cs + off];
-    C g4 = G[4 * cs + off];
-    const C g5 = G[5 * cs + off];
-    if (sx * sy < 0) g1 = cscale(g1, -1);
-    if (sx * sz < 0) g2 = cscale(g2, -1);
-    if (sy * sz < 0) g4 = cscale(g4, -1);
+    C g[6];
+    green_load6(G, off, g);
+    if (sx * sy < 0) g[1] = cscale(g[1], -1);
+    if (sx * sz < 0) g[2] = cscale(g[2], -1);
+    if (sy * sz < 0) g[4] = cscale(g[4], -1);
     const C a0 = v0, a1 = v1, a2 = v2;
-    v0 = cadd(cadd(cmul(g0, a0), cmul(g1, a1)), cmul(g2, a2));
-    v1 = cadd(cadd(cmul(g1, a0), cmul(g3, a1)), cmul(g4, a2));
-    v2 = cadd(cadd(cmul(g2, a0), cmul(g4, a1)), cmul(g5, a2));
+    v0 = cadd(cadd(cmul(g[0], a0), cmul(g[1], a1)), cmul(g[2], a2));
+    v1 = cadd(cadd(cmul(g[1], a0), cmul(g[3], a1)), cmul(g[4], a2));
+    v2 = cadd(cadd(cmul(g[2], a0), cmul(g[4], a1)), cmul(g[5], a2));
 }
 
 }  // namespace tdda

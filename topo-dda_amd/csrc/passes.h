@@ -62,8 +62,10 @@ void launch_matvec(const LaunchSet& ls, const TwiddlePools& tw, bool c64, int xf
 // `st`; adds the elapsed ms of X fwd, Y fwd, Z, Y inv, X inv to ms5.
 void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, hipStream_t st, int reps,
                          double* ms5);
-// Elementwise BiCGSTAB update and convergence logic.
-void launch_update(const LaunchSet& ls, int gate, hipStream_t st);
+// The end-of-step vector update of a BiCGSTAB step is applied by the X
+// forward pass of the next matvec; jobs that stop at a step's decision still
+// owe it: call after every batch of iterations (no-op for the others).
+void launch_finalize(const LaunchSet& ls, hipStream_t st);
 // Initial residual, r0, p = v = 0 and scalars; warm: r = rhs - yout.
 void launch_init(const LaunchSet& ls, bool warm, hipStream_t st);
 

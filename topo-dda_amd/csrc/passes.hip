@@ -106,7 +106,7 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
 
     const Real inv_m = (Real)(1.0 / ((double)J.lx * (double)J.ly * (double)J.lz));
     const C* al = Tile<Real>::al(J);
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[XI_NACC] = {};
     const int nel = 3 * nx * nb;
     for (int e = threadIdx.x; e < nel; e += FFT_THREADS) {
         const int c = e % 3;
@@ -279,63 +279,34 @@ k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const c
 }
 
 // ----------------------------------------------------------------------------
-// BiCGSTAB update (DDAModel.cpp:981-986):  z = zeta r - alpha u, u = zeta v,
-// P += alpha p + z;  r <- t - zeta s;  partial <r0, r>, ||r||^2
+// Pending end-of-step update of jobs that stopped at the XI_S decision
+// (converged, iteration limit or breakdown with the step's P and r update
+// still owed): P += alpha p + zeta t, r = t - zeta s.  The last block of a
+// job clears the flag, so a second launch changes nothing.
 // ----------------------------------------------------------------------------
 template <typename KReal>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_update(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int gate) {
+k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
     using KC = cx_t<KReal>;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
-    if (!gate_open(J, gate)) return;
     KScalars* ks = J.ks;
-    const double2 alpha = ks->alpha, zeta = ks->zeta;
+    if (ks == nullptr || ks->status == JOB_ACTIVE || ks->pending == 0) return;
+    XfScalars sc{ks->alpha, ks->beta, ks->zeta, 1};
     const long n3 = 3L * J.ndip;
     const long base = (long)jt.y * FFT_THREADS * UPD_EPT;
     const KC* p = static_cast<const KC*>(J.p);
-    KC* r = static_cast<KC*>(J.r);
-    const KC* v = static_cast<const KC*>(J.v);
-    const KC* t = static_cast<const KC*>(J.t);
-    const KC* s = static_cast<const KC*>(J.s);
-    const KC* r0 = static_cast<const KC*>(J.r0);
-    double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < UPD_EPT; ++k) {
         const long i = base + (long)k * FFT_THREADS + threadIdx.x;
-        if (i < n3) {
-            const double2 rv = to_d(r[i]);
-            const double2 u = dcmul(zeta, to_d(v[i]));
-            const double2 z = csub(dcmul(zeta, rv), dcmul(alpha, u));
-            J.P[i] = cadd(cadd(J.P[i], dcmul(alpha, to_d(p[i]))), z);
-            const KC rk = from_d<KC>(csub(to_d(t[i]), dcmul(zeta, to_d(s[i]))));
-            r[i] = rk;
-            const double2 rd = to_d(rk);
-            const double2 d = cmulc(rd, to_d(r0[i]));
-            acc[0] += d.x;
-            acc[1] += d.y;
-            acc[2] += rd.x * rd.x + rd.y * rd.y;
-        }
+        if (i < n3) apply_pending<KC>(J, i, sc, to_d(p[i]), true);
     }
-    block_sum<3>(acc);
-    if (!publish_partials<3>(J, jt.y, J.tupd, acc)) return;
-    double sum[3];
-    sum_partials<3>(J, J.tupd, sum);
-    if (threadIdx.x != 0) return;
-    const double2 rho_new = make_double2(sum[0], sum[1]);
-    const double rn2 = sum[2];
-    if (ks->refined) {
-        const double rel2 = rn2 / ks->enorm2;
-        const bool trigger = rn2 <= ks->ru_delta * ks->ru_delta * ks->maxr2 ||
-                             rel2 <= ks->tol * ks->tol || ks->iter + 1 >= ks->maxit;
-        if (trigger && ks->iter >= 0) {
-            ks->need_ru = 1;     // true residual decides (k_xinv XI_RESID)
-            ks->rnorm2 = rn2;
-            return;
-        }
-        if (rn2 > ks->maxr2) ks->maxr2 = rn2;
+    const double none[1] = {0.0};
+    if (!publish_partials<1>(J, jt.y, J.tupd, none)) return;
+    if (threadIdx.x == 0) {
+        ks->pending = 0;
+        __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    bicg_decide(ks, rho_new, rn2);
 }
 
 // ----------------------------------------------------------------------------
@@ -384,6 +355,7 @@ k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm
     ks->maxr2 = sum[0];
     ks->iter = -1;
     ks->need_ru = 0;
+    ks->pending = 0;
     ks->n_ru = 0;
     ks->iterations = 0;
     ks->err = 0.0;
@@ -519,12 +491,12 @@ void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, 
         matvec_timed_impl<double, double>(ls, tw, st, reps, ms5);
 }
 
-void launch_update(const LaunchSet& ls, int gate, hipStream_t st) {
+void launch_finalize(const LaunchSet& ls, hipStream_t st) {
     if (ls.n_upd == 0) return;
     if (ls.refined)
-        hipLaunchKernelGGL((k_update<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, gate);
+        hipLaunchKernelGGL((k_finalize<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd);
     else
-        hipLaunchKernelGGL((k_update<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, gate);
+        hipLaunchKernelGGL((k_finalize<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd);
     TDDA_HIP(hipGetLastError());
 }
 

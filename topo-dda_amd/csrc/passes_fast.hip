@@ -67,7 +67,7 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[XI_NACC] = {};
     with_split(J.lx, [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         using X = XFast<L1, L2, fast_nr_x<Real>(L1, L2), C>;

@@ -246,8 +246,9 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     launch_green_octant(P, tables.si, tables.ci, a.get(), st);
     launch_green_dft(a.get(), b.get(), nz * ny, nx, hx, 1, lx, 0, tw.tw64 + px.tw_off, st);
     launch_green_dft(b.get(), c.get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
-    launch_green_dft(c.get(), green64.get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
-    launch_to_c64(green64.get(), green32.get(), (long)green64.size(), st);
+    DevBuf<double2> e(6L * hz * hy * hx);
+    launch_green_dft(c.get(), e.get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
+    launch_green_interleave(e.get(), (long)hz * hy * hx, green64.get(), green32.get(), st);
     TDDA_HIP(hipStreamSynchronize(st));
     setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -362,7 +363,7 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.px = o.px; job.py = o.py; job.pz = o.pz;
     plan_tiles(job);
     const long nparts = std::max({(long)job.tupd, (long)job.tx32, (long)job.tx64});
-    partial.alloc(nparts * 6);
+    partial.alloc(nparts * PARTIAL_STRIDE);
     if (box_map_host) {
         box_map.alloc((size_t)o.nx * o.ny * o.nz);
         box_map.upload(box_map_host, box_map.size(), st);
@@ -603,9 +604,9 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
         for (int k = 0; k < chunk; ++k) {
             launch_matvec(b.ls, tw, c64, XF_P, XI_V, GATE_ACTIVE, st);
             launch_matvec(b.ls, tw, c64, XF_T, XI_S, GATE_ACTIVE, st);
-            launch_update(b.ls, GATE_ACTIVE, st);
             if (opt.refined) launch_matvec(b.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, st);
         }
+        launch_finalize(b.ls, st);
         done_iters += chunk;
         ks.download(snap.data(), nj, st);
         TDDA_HIP(hipStreamSynchronize(st));

@@ -197,9 +197,9 @@ void Server::loop() {
                 for (int k = 0; k < opt_.chunk; ++k) {
                     launch_matvec(batch_.ls, tw, opt_.refined, XF_P, XI_V, GATE_ACTIVE, main_);
                     launch_matvec(batch_.ls, tw, opt_.refined, XF_T, XI_S, GATE_ACTIVE, main_);
-                    launch_update(batch_.ls, GATE_ACTIVE, main_);
                     if (opt_.refined) launch_matvec(batch_.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, main_);
                 }
+                launch_finalize(batch_.ls, main_);
                 TDDA_HIP(hipMemcpyAsync(ks_host_.data(), ks_.get(), ks_host_.size() * sizeof(KScalars),
                                         hipMemcpyDeviceToHost, main_));
                 ran = true;
@@ -212,7 +212,7 @@ void Server::loop() {
                 std::lock_guard<std::mutex> lk(mu_);
                 if (ran) {
                     stats_.chunks++;
-                    stats_.launches += (long)opt_.chunk * (opt_.refined ? 16 : 11);
+                    stats_.launches += (long)opt_.chunk * (opt_.refined ? 15 : 10) + 1;
                 }
                 stats_.peak_bytes = std::max(stats_.peak_bytes, device_bytes_in_use());
             }
