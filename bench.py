@@ -137,17 +137,18 @@ def cpu_baseline(reqs, iterations, per_shape_s=4.0):
 
 
 PROBE_PER_SHAPE = 8
-TRAFFIC_FILE = "k_zpass_probe40_pmc.md"
+TRAFFIC_FILE = "matvec_probe40_pmc.md"
 
 
-def zpass_traffic():
-    """HBM bytes per k_zpass launch on the probe batch, from the committed
-    rocprofv3 counter summary (counters cannot be read inside this process)."""
+def matvec_traffic():
+    """HBM bytes per pass launch (X fwd, Y fwd, Z, Y inv, X inv) on the probe
+    batch, from the committed rocprofv3 counter summary (counters cannot be
+    read inside this process); None if absent."""
     try:
         with open(os.path.join(ROOT, "profiles", TRAFFIC_FILE)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                if line.startswith("pass_traffic_bytes:"):
+                    return [int(x) for x in line.split(":")[1].split(",")]
     except OSError:
         pass
     return None
@@ -165,11 +166,50 @@ def roofline_probe(native, reps):
     probs = [native.Problem(native.Request(shape=s, d=15.0, lam=500.0 + 3.0 * i, rho=np.full(int(np.prod(s)), 0.5)),
                             precision="fp32_refined") for i, s in enumerate(probe_shapes())]
     ms5 = native.time_passes(probs, c64=True, reps=reps)
+    ms10 = native.time_iteration(probs, reps=reps)
     src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")   # 256 MiB
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    return ms5, probs
+    return ms5, ms10, probs
+
+
+def roofline_record(ms5):
+    """Roofline of one batched matvec y = A x + al x over the probe batch.
+
+    Algorithmic bytes per matvec and problem (the hot path's unit): with s = 8
+    (c64), n = 3 Nx Ny Nz unknowns and M = Lx Ly Lz the padded grid,
+    B_mv = s (2n + 12 M) + 6 s M: x in and y out, one read and one write of
+    the 3-component grid per transform direction, and the 6-component Green's
+    spectrum once.  `achieved` is the batch's sum of B_mv over the five pass
+    launches' summed duration (HIP events on the launch stream).  The passes
+    move less than B_mv (pruned zero padding, octant spectrum), so the
+    measured HBM bytes are reported beside it as `traffic` and `actual_GBs`."""
+    Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in probe_shapes()]
+    ns = [3 * int(np.prod(s)) for s in probe_shapes()]
+    b_mv = sum(8 * (2 * n + 12 * M) + 8 * 6 * M for n, M in zip(ns, Ms))
+    t = float(sum(ms5)) * 1e-3
+    pt = matvec_traffic()
+    names = ["x_fwd", "y_fwd", "z", "y_inv", "x_inv"]
+    passes = {}
+    for k, nm in enumerate(names):
+        e = {"ms": round(float(ms5[k]), 5)}
+        if pt:
+            e["bytes"] = pt[k]
+            e["GBs"] = round(pt[k] / (ms5[k] * 1e-3) / 1e9, 1)
+        passes[nm] = e
+    traffic = sum(pt) if pt else None
+    return {"bound": "hbm",
+            "kernel": f"batched matvec y = A x + al x, c64: k_xfwd_fast, k_ypass_fast<-1>, k_zpass_fast, "
+                      f"k_ypass_fast<+1>, k_xinv_fast, one launch each over a {len(Ms)}-request batch "
+                      f"({PROBE_PER_SHAPE} per box shape)",
+            "achieved": round(b_mv / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b_mv / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes": b_mv, "matvec_ms": round(t * 1e3, 5),
+            "actual_GBs": round(traffic / t / 1e9, 1) if traffic else None,
+            "actual_frac": round(traffic / t / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+            "traffic_source": f"profiles/{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, same batch)",
+            "passes": passes}
 
 
 def main():
@@ -185,6 +225,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the batched pass timing and a 256 MiB copy (for counter runs)")
+    ap.add_argument("--saturation-only", action="store_true",
+                    help="only warm-up and the timed saturation steps; prints solves/s (for kernel traces)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (no multi-GPU path is built)",
@@ -193,8 +235,10 @@ def main():
     from dda_model import native
 
     if a.roofline_only:
-        ms5, _ = roofline_probe(native, 30)
-        print(json.dumps({"pass_ms": [float(x) for x in ms5], "copy_bytes": 2 * (256 << 20)}), flush=True)
+        ms5, ms10, _ = roofline_probe(native, 30)
+        print(json.dumps({"pass_ms": [float(x) for x in ms5], "iteration_pass_ms": [float(x) for x in ms10],
+                          "copy_bytes": 2 * (256 << 20),
+                          "roofline": roofline_record(ms5)}), flush=True)
         return
 
     reqs = make_requests(native, a.requests)
@@ -214,6 +258,10 @@ def main():
     total = sum(walls)
     value = a.steps * len(reqs) / total
     conv = sum(1 for r in res_sat.values() if r["status"] == "converged")
+    if a.saturation_only:
+        print(json.dumps({"saturation_solves_per_s": round(value, 3), "converged": conv,
+                          "server": stats_sat}), flush=True)
+        return
 
     # Poisson arrivals at load x saturation -> latency
     rate = a.load * value
@@ -232,21 +280,10 @@ def main():
     e_err = max(float(np.linalg.norm(f32[i] - f64[i]) / np.linalg.norm(f64[i])) for i in f64)
     print(f"# fp64 single-solve: {fp64_rate:.2f} solves/s", file=sys.stderr, flush=True)
 
-    # roofline: one batched launch per pass over one problem of every shape
-    ms5, probs = roofline_probe(native, 30)
-    Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in probe_shapes()]
-    ns = [3 * int(np.prod(s)) for s in probe_shapes()]
-    b_z = sum(8 * (4 * M + 6 * M) for M in Ms)
-    b_mv = sum(8 * (2 * n + 12 * M) + 8 * 6 * M for n, M in zip(ns, Ms))
-    roof = {"bound": "hbm", "kernel": "k_zpass<float>: z FFT + Green multiply + inverse z FFT, one launch "
-                                      f"over a {len(Ms)}-request batch ({PROBE_PER_SHAPE} per shape)",
-            "achieved": round(b_z / (ms5[2] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b_z / (ms5[2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": zpass_traffic(),
-            "traffic_source": f"profiles/{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, same batch)",
-            "algorithmic_bytes": b_z, "z_ms": round(float(ms5[2]), 5),
-            "matvec_ms": round(float(sum(ms5)), 5),
-            "matvec_achieved_GBs": round(b_mv / (sum(ms5) * 1e-3) / 1e9, 1),
-            "pass_ms": [round(float(x), 5) for x in ms5]}
+    # roofline of the batched matvec (one launch per pass over the probe batch)
+    ms5, ms10, probs = roofline_probe(native, 30)
+    roof = roofline_record(ms5)
+    roof["iteration_pass_ms"] = [round(float(x), 5) for x in ms10]
     del probs
 
     cpu = None

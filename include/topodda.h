@@ -192,6 +192,11 @@ int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* ta
  * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
  * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */
 int tdda_problems_time_passes(tdda_problem** ps, int n, int c64, int reps, double* ms5);
+/* Average ms per launch of the ten passes of one refined BiCGSTAB step over
+ * all n problems (TDDA_PREC_FP32_REFINED): X fwd with the p update, Y, Z, Y,
+ * X inv with <r0, v>; X fwd with t, Y, Z, Y, X inv with the step's dots.
+ * Starts from P = 0 and overwrites the problems' solver state. */
+int tdda_problems_time_iteration(tdda_problem** ps, int n, int reps, double* ms10);
 
 typedef struct {
     int precision;       /* TDDA_PREC_* */

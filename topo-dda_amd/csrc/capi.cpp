@@ -421,6 +421,18 @@ int tdda_problems_time_passes(tdda_problem** ps, int n, int c64, int reps, doubl
     });
 }
 
+int tdda_problems_time_iteration(tdda_problem** ps, int n, int reps, double* ms10) {
+    return guarded([&] {
+        TDDA_REQUIRE(ps != nullptr && n >= 1 && ms10 != nullptr, tdda::ERR_INVALID, "no problems");
+        std::vector<tdda::System*> sys;
+        for (int i = 0; i < n; ++i) {
+            TDDA_REQUIRE(ps[i] && ps[i]->p, tdda::ERR_INVALID, "null problem");
+            sys.push_back(ps[i]->p->sys.get());
+        }
+        tdda::time_iteration_batch(sys, reps, tdda::default_stream(), ms10);
+    });
+}
+
 int tdda_server_create(const tdda_server_opts* o, const double* si, const double* ci, long nsici, double delta,
                        tdda_server** out) {
     return guarded([&] {

@@ -70,7 +70,6 @@ struct GridJob {
     void* yout;
     KScalars* ks;
     double* partial;           // [max tiles][PARTIAL_STRIDE]
-    unsigned* counter;
 };
 
 }  // namespace tdda

@@ -60,41 +60,35 @@ __device__ inline void block_sum(double (&v)[NV]) {
     __syncthreads();   // red may be reused by the next call
 }
 
-// Publish this block's partial sums; returns true (in every thread) for the
-// block that completes the job.  That block then sums all partials in tile
-// order (fixed, hence deterministic) through sum_partials().
+// Cross-block sums go through a per-tile partial buffer: every block of a
+// pass writes its block sums (write_partials), and a one-block-per-job kernel
+// launched after the pass adds them in tile order (job_sum) and advances the
+// job's scalars.  The kernel boundary makes the partials visible; no block
+// waits on another, and no device-scope fence is issued per block (on a
+// multi-XCD part that fence writes back the XCD's whole L2).
 template <int NV>
-__device__ inline bool publish_partials(const GridJob& J, int tile, int ntiles, const double (&v)[NV]) {
-    __shared__ int s_last;
+__device__ inline void write_partials(const GridJob& J, int tile, const double (&v)[NV]) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) J.partial[(long)tile * PARTIAL_STRIDE + k] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(J.counter, 1u);
-        s_last = (prev + 1 == (unsigned)ntiles) ? 1 : 0;
     }
-    __syncthreads();
-    return s_last != 0;
 }
 
-// Called by every thread of the completing block: thread t sums tiles t,
-// t+256, ... and the block tree-reduces, a fixed order for a given job shape.
+// Called by every thread of a one-block-per-job kernel: thread t sums tiles
+// t, t+256, ... and the block reduces, a fixed order for a given job shape.
 template <int NV>
-__device__ inline void sum_partials(const GridJob& J, int ntiles, double (&out)[NV]) {
-    __threadfence();
+__device__ inline void job_sum(const GridJob& J, int ntiles, double (&out)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = 0.0;
     for (int t = threadIdx.x; t < ntiles; t += FFT_THREADS) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            out[k] += __hip_atomic_load(&J.partial[(long)t * PARTIAL_STRIDE + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < NV; ++k) out[k] += J.partial[(long)t * PARTIAL_STRIDE + k];
     }
     block_sum<NV>(out);
-    if (threadIdx.x == 0) __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ----------------------------------------------------------------------------
-// scalar logic (runs in one thread of the last block), DDAModel.cpp:945-1004
+// scalar logic (one thread of the per-job reduction), DDAModel.cpp:945-1004
 // ----------------------------------------------------------------------------
 __device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2) {
     ks->rnorm2 = rn2;
@@ -288,8 +282,15 @@ __device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __
     }
 }
 
-// Reduce the dot partials of a job over its X tiles and advance its scalars
-// in the completing block (all threads call it).
+// Block sums of the dot partials of an X inverse tile (all threads call it).
+template <int MODE>
+__device__ inline void xi_finish(const GridJob& J, int tile, double (&acc)[XI_NACC]) {
+    if (MODE == XI_PLAIN) return;
+    block_sum<XI_NACC>(acc);
+    write_partials<XI_NACC>(J, tile, acc);
+}
+
+// Advance a job's scalars from the summed dot partials (one thread):
 //   XI_V     : alpha = <r0,r>/<r0,Ap>; the pending update was applied by the
 //              X forward pass of this matvec
 //   XI_S     : zeta = <At,t>/<At,At>; with r = t - zeta s,
@@ -299,14 +300,7 @@ __device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __
 //   XI_RESID : true residual of the updated P (the X forward pass applied the
 //              pending update): reliable-update bookkeeping and decision
 template <int MODE>
-__device__ inline void xi_finish(const GridJob& J, int tile, int ntiles, double (&acc)[XI_NACC]) {
-    if (MODE == XI_PLAIN) return;
-    block_sum<XI_NACC>(acc);
-    if (!publish_partials<XI_NACC>(J, tile, ntiles, acc)) return;
-    double sum[XI_NACC];
-    sum_partials<XI_NACC>(J, ntiles, sum);
-    if (threadIdx.x != 0) return;
-    KScalars* ks = J.ks;
+__device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC]) {
     const double2 dot = make_double2(sum[0], sum[1]);
     if (MODE == XI_V) {
         ks->alpha = ddiv(ks->rho, dot);

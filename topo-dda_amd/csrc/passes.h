@@ -42,6 +42,7 @@ enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
 // length has a compile-time split).
 struct LaunchSet {
     const GridJob* jobs = nullptr;   // device
+    int njobs = 0;
     PassMap pm[2][3][2];             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
     const int2* map_upd = nullptr;
     int n_upd = 0;
@@ -61,7 +62,7 @@ void launch_matvec(const LaunchSet& ls, const TwiddlePools& tw, bool c64, int xf
 // Same launches as launch_matvec (plain mode), each bracketed by HIP events on
 // `st`; adds the elapsed ms of X fwd, Y fwd, Z, Y inv, X inv to ms5.
 void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, hipStream_t st, int reps,
-                         double* ms5);
+                         double* ms5, int xf_mode = XF_PLAIN, int xi_mode = XI_PLAIN);
 // The end-of-step vector update of a BiCGSTAB step is applied by the X
 // forward pass of the next matvec; jobs that stop at a step's decision still
 // owe it: call after every batch of iterations (no-op for the others).

@@ -117,7 +117,7 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
         if (vi < 0) continue;
         xi_element<C, KC, MODE>(J, vi, cscale(lds[x * bp + 3 * b + c], inv_m), al, acc);
     }
-    xi_finish<MODE>(J, jt.y, Tile<Real>::tx(J), acc);
+    xi_finish<MODE>(J, jt.y, acc);
 }
 
 // ----------------------------------------------------------------------------
@@ -281,8 +281,8 @@ k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const c
 // ----------------------------------------------------------------------------
 // Pending end-of-step update of jobs that stopped at the XI_S decision
 // (converged, iteration limit or breakdown with the step's P and r update
-// still owed): P += alpha p + zeta t, r = t - zeta s.  The last block of a
-// job clears the flag, so a second launch changes nothing.
+// still owed): P += alpha p + zeta t, r = t - zeta s.  k_finalize_clear,
+// launched after it, clears the flag, so a second pair changes nothing.
 // ----------------------------------------------------------------------------
 template <typename KReal>
 __global__ void __launch_bounds__(FFT_THREADS)
@@ -290,7 +290,7 @@ k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
     using KC = cx_t<KReal>;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
-    KScalars* ks = J.ks;
+    const KScalars* ks = J.ks;
     if (ks == nullptr || ks->status == JOB_ACTIVE || ks->pending == 0) return;
     XfScalars sc{ks->alpha, ks->beta, ks->zeta, 1};
     const long n3 = 3L * J.ndip;
@@ -301,12 +301,28 @@ k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
         const long i = base + (long)k * FFT_THREADS + threadIdx.x;
         if (i < n3) apply_pending<KC>(J, i, sc, to_d(p[i]), true);
     }
-    const double none[1] = {0.0};
-    if (!publish_partials<1>(J, jt.y, J.tupd, none)) return;
-    if (threadIdx.x == 0) {
-        ks->pending = 0;
-        __hip_atomic_store(J.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+}
+
+// one thread per job
+__global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    KScalars* ks = jobs[j].ks;
+    if (ks != nullptr && ks->status != JOB_ACTIVE) ks->pending = 0;
+}
+
+// ----------------------------------------------------------------------------
+// per-job sums of the X-inverse dot partials and the scalar step (one block
+// per job, after the X inverse launches of a matvec)
+// ----------------------------------------------------------------------------
+template <typename Real, int MODE>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_xi_reduce(const GridJob* __restrict__ jobs, int gate) {
+    const GridJob& J = jobs[blockIdx.x];
+    if (!gate_open(J, gate)) return;
+    double sum[XI_NACC];
+    job_sum<XI_NACC>(J, Tile<Real>::tx(J), sum);
+    if (threadIdx.x == 0) xi_decide<MODE>(J.ks, sum);
 }
 
 // ----------------------------------------------------------------------------
@@ -340,9 +356,15 @@ k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm
         }
     }
     block_sum<2>(acc);
-    if (!publish_partials<2>(J, jt.y, J.tupd, acc)) return;
+    write_partials<2>(J, jt.y, acc);
+}
+
+template <typename KReal>
+__global__ void __launch_bounds__(FFT_THREADS)
+k_init_reduce(const GridJob* __restrict__ jobs, int warm) {
+    const GridJob& J = jobs[blockIdx.x];
     double sum[2];
-    sum_partials<2>(J, J.tupd, sum);
+    job_sum<2>(J, J.tupd, sum);
     if (threadIdx.x != 0) return;
     KScalars* ks = J.ks;
     ks->alpha = make_double2(0, 0);
@@ -434,6 +456,15 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
         default:
             xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
             launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, st);
+            if (xi != XI_PLAIN && ls.njobs > 0) {
+                const dim3 grid(ls.njobs), blk(FFT_THREADS);
+                switch (xi) {
+                    case XI_V: hipLaunchKernelGGL((k_xi_reduce<Real, XI_V>), grid, blk, 0, st, ls.jobs, gate); break;
+                    case XI_S: hipLaunchKernelGGL((k_xi_reduce<Real, XI_S>), grid, blk, 0, st, ls.jobs, gate); break;
+                    case XI_RESID: hipLaunchKernelGGL((k_xi_reduce<Real, XI_RESID>), grid, blk, 0, st, ls.jobs, gate); break;
+                    default: throw Error(ERR_INVALID, "bad X-inverse mode");
+                }
+            }
             break;
     }
 }
@@ -461,13 +492,14 @@ void launch_matvec(const LaunchSet& ls, const TwiddlePools& tw, bool c64, int xf
 }
 
 template <typename Real, typename KReal>
-static void matvec_timed_impl(const LaunchSet& ls, const TwiddlePools& tw, hipStream_t st, int reps, double* ms5) {
+static void matvec_timed_impl(const LaunchSet& ls, const TwiddlePools& tw, hipStream_t st, int reps, double* ms5,
+                              int xf, int xi) {
     hipEvent_t ev[6];
     for (auto& e : ev) TDDA_HIP(hipEventCreate(&e));
     for (int r = 0; r < reps; ++r) {
         TDDA_HIP(hipEventRecord(ev[0], st));
         for (int k = 0; k < 5; ++k) {
-            pass_impl<Real, KReal>(k, ls, tw, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+            pass_impl<Real, KReal>(k, ls, tw, xf, xi, GATE_ALWAYS, st);
             TDDA_HIP(hipEventRecord(ev[k + 1], st));
         }
         TDDA_HIP(hipEventSynchronize(ev[5]));
@@ -482,13 +514,13 @@ static void matvec_timed_impl(const LaunchSet& ls, const TwiddlePools& tw, hipSt
 }
 
 void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, hipStream_t st, int reps,
-                         double* ms5) {
+                         double* ms5, int xf_mode, int xi_mode) {
     if (c64)
-        matvec_timed_impl<float, float>(ls, tw, st, reps, ms5);
+        matvec_timed_impl<float, float>(ls, tw, st, reps, ms5, xf_mode, xi_mode);
     else if (ls.refined)
-        matvec_timed_impl<double, float>(ls, tw, st, reps, ms5);
+        matvec_timed_impl<double, float>(ls, tw, st, reps, ms5, xf_mode, xi_mode);
     else
-        matvec_timed_impl<double, double>(ls, tw, st, reps, ms5);
+        matvec_timed_impl<double, double>(ls, tw, st, reps, ms5, xf_mode, xi_mode);
 }
 
 void launch_finalize(const LaunchSet& ls, hipStream_t st) {
@@ -497,6 +529,7 @@ void launch_finalize(const LaunchSet& ls, hipStream_t st) {
         hipLaunchKernelGGL((k_finalize<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd);
     else
         hipLaunchKernelGGL((k_finalize<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd);
+    hipLaunchKernelGGL(k_finalize_clear, dim3((ls.njobs + 255) / 256), dim3(256), 0, st, ls.jobs, ls.njobs);
     TDDA_HIP(hipGetLastError());
 }
 
@@ -506,6 +539,10 @@ void launch_init(const LaunchSet& ls, bool warm, hipStream_t st) {
         hipLaunchKernelGGL((k_init<float>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, (int)warm);
     else
         hipLaunchKernelGGL((k_init<double>), dim3(ls.n_upd), dim3(FFT_THREADS), 0, st, ls.jobs, ls.map_upd, (int)warm);
+    if (ls.refined)
+        hipLaunchKernelGGL((k_init_reduce<float>), dim3(ls.njobs), dim3(FFT_THREADS), 0, st, ls.jobs, (int)warm);
+    else
+        hipLaunchKernelGGL((k_init_reduce<double>), dim3(ls.njobs), dim3(FFT_THREADS), 0, st, ls.jobs, (int)warm);
     TDDA_HIP(hipGetLastError());
 }
 

@@ -97,7 +97,7 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
                 if (vi >= 0) xi_element<C, KC, MODE>(J, vi, cscale(v, inv_m), al, acc);
             });
     });
-    xi_finish<MODE>(J, jt.y, Tile<Real>::tx(J), acc);
+    xi_finish<MODE>(J, jt.y, acc);
 }
 
 // ----------------------------------------------------------------------------

@@ -352,8 +352,6 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     scratch64.alloc(n3);
     al64.alloc(ndip);
     al32.alloc(ndip);
-    counter.alloc(1);
-    counter.zero(st);
     P.zero(st);
 
     std::memset(&job, 0, sizeof job);
@@ -388,13 +386,12 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.P = P.get();
     job.rhs = rhs.get();
     job.partial = partial.get();
-    job.counter = counter.get();
     job.ks = nullptr;
 }
 
 size_t System::bytes() const {
     return grid.bytes() + kvec.bytes() + P.bytes() + rhs.bytes() + scratch64.bytes() + al64.bytes() +
-           al32.bytes() + partial.bytes() + counter.bytes() + box_map.bytes();
+           al32.bytes() + partial.bytes() + box_map.bytes();
 }
 
 namespace {
@@ -545,6 +542,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     const int2* base = maps_.get();
     ls = LaunchSet{};
     ls.jobs = djobs_.get();
+    ls.njobs = (int)hjobs.size();
     for (int pr = 0; pr < 2; ++pr)
         for (int p = 0; p < 3; ++p)
             for (int k = 0; k < 2; ++k) {
@@ -690,6 +688,38 @@ void time_passes_batch(const std::vector<System*>& v, bool c64, int reps, hipStr
     for (int k = 0; k < 5; ++k) ms5[k] = 0.0;
     launch_matvec_timed(l, PlanRegistry::instance().pools(), c64, st, reps, ms5);
     for (int k = 0; k < 5; ++k) ms5[k] /= reps;
+}
+
+void time_iteration_batch(const std::vector<System*>& v, int reps, hipStream_t st, double* ms10) {
+    TDDA_REQUIRE(!v.empty(), ERR_INVALID, "no systems");
+    TDDA_REQUIRE(reps >= 1, ERR_INVALID, "reps must be positive");
+    for (System* s : v) TDDA_REQUIRE(s->refined, ERR_INVALID, "iteration timing needs refined systems");
+    // a real solve state: P = 0, r = r0 = rhs, then the timed steps
+    DevBuf<KScalars> ks(v.size());
+    std::vector<KScalars> hks(v.size());
+    std::memset(hks.data(), 0, hks.size() * sizeof(KScalars));
+    for (auto& k : hks) {
+        k.tol = 1e-30;
+        k.maxit = 1 << 30;
+        k.refined = 1;
+        k.ru_delta = 0.0;
+    }
+    ks.upload(hks.data(), hks.size(), st);
+    for (System* s : v) TDDA_HIP(hipMemsetAsync(s->P.get(), 0, s->P.bytes(), st));
+    Batch b;
+    b.build(v, ks.get(), true, st);
+    launch_init(b.ls, false, st);
+    const TwiddlePools tw = PlanRegistry::instance().pools();
+    double warm[5] = {0, 0, 0, 0, 0};
+    launch_matvec_timed(b.ls, tw, true, st, 1, warm, XF_P, XI_V);
+    launch_matvec_timed(b.ls, tw, true, st, 1, warm, XF_T, XI_S);
+    for (int k = 0; k < 10; ++k) ms10[k] = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        launch_matvec_timed(b.ls, tw, true, st, 1, ms10, XF_P, XI_V);
+        launch_matvec_timed(b.ls, tw, true, st, 1, ms10 + 5, XF_T, XI_S);
+    }
+    for (int k = 0; k < 10; ++k) ms10[k] /= reps;
+    TDDA_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace tdda

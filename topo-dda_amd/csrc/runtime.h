@@ -169,7 +169,6 @@ class System {
     DevBuf<double2> al64;
     DevBuf<float2> al32;
     DevBuf<double> partial;
-    DevBuf<unsigned> counter;
     DevBuf<int> box_map;
 };
 
@@ -223,6 +222,10 @@ void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st);
 void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5);
 // same over a batch of systems in one launch per pass
 void time_passes_batch(const std::vector<System*>& v, bool c64, int reps, hipStream_t st, double* ms5);
+// Average ms per launch of the ten passes of one refined BiCGSTAB step
+// (X fwd with p update ... X inv with <r0,v>, then X fwd with t ... X inv with
+// the XI_S dots) over a batch, on zeroed Krylov state.
+void time_iteration_batch(const std::vector<System*>& v, int reps, hipStream_t st, double* ms10);
 
 hipStream_t default_stream();
 

@@ -70,6 +70,7 @@ def lib():
         L.tdda_problem_gradient.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(SolveInfo)]
         L.tdda_problems_time_passes.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_problems_time_iteration.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, dp]
         L.tdda_server_create.argtypes = [ctypes.POINTER(ServerOpts), dp, dp, ctypes.c_long, ctypes.c_double,
                                          ctypes.POINTER(vp)]
         L.tdda_server_destroy.argtypes = [vp]
@@ -220,6 +221,15 @@ def time_passes(problems, c64=True, reps=20):
     arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
     ms = np.zeros(5)
     check(lib().tdda_problems_time_passes(arr, len(problems), int(c64), reps, _dp(ms)))
+    return ms
+
+
+def time_iteration(problems, reps=20):
+    """Average ms per batched launch of the ten passes of one refined
+    BiCGSTAB step (p-update matvec, then t matvec); overwrites solver state."""
+    arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
+    ms = np.zeros(10)
+    check(lib().tdda_problems_time_iteration(arr, len(problems), reps, _dp(ms)))
     return ms
 
 
