@@ -7,6 +7,30 @@
 
 namespace tdda {
 
+// Global-memory accesses through pointers read from job descriptors: without
+// an address-space cast the compiler cannot tell global from LDS and emits
+// FLAT accesses, which also occupy the LDS counter and serialise waits.  The
+// HIP vector types cannot be copied out of address space 1 directly, so the
+// accessors move the same bytes as native clang vectors.  Offsets are 32-bit
+// (every array of one job is far below 2^31 elements).
+#define TDDA_GLOBAL __attribute__((address_space(1)))
+template <typename T> struct NativeVec { using type = T; };
+template <> struct NativeVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct NativeVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NativeVec<float4> { typedef float type __attribute__((ext_vector_type(4))); };
+
+template <typename T>
+__device__ __forceinline__ T gload(const void* base, unsigned i) {
+    using V = typename NativeVec<T>::type;
+    const V v = ((const TDDA_GLOBAL V*)base)[i];
+    return __builtin_bit_cast(T, v);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(void* base, unsigned i, T x) {
+    using V = typename NativeVec<T>::type;
+    ((TDDA_GLOBAL V*)base)[i] = __builtin_bit_cast(V, x);
+}
+
 template <typename Real> struct Tile;
 template <> struct Tile<float> {
     __device__ static int bx(const GridJob& J) { return J.bx32; }
@@ -167,63 +191,59 @@ __device__ inline XfScalars xf_scalars(const GridJob& J) {
 
 // the pending step update of one element; returns the new P
 template <typename KC>
-__device__ inline double2 apply_pending(const GridJob& J, long vi, const XfScalars& sc, double2 pv, bool write_r) {
-    const double2 td = to_d(static_cast<const KC*>(J.t)[vi]);
-    const double2 Pn = cadd(J.P[vi], cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
-    J.P[vi] = Pn;
+__device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfScalars& sc, double2 pv, bool write_r) {
+    const double2 td = to_d(gload<KC>(J.t, vi));
+    const double2 Pn = cadd(gload<double2>(J.P, vi), cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
+    gstore<double2>(J.P, vi, Pn);
     if (write_r) {
-        const double2 sd = to_d(static_cast<const KC*>(J.s)[vi]);
-        static_cast<KC*>(J.r)[vi] = from_d<KC>(csub(td, dcmul(sc.zeta, sd)));
+        const double2 sd = to_d(gload<KC>(J.s, vi));
+        gstore<KC>(J.r, vi, from_d<KC>(csub(td, dcmul(sc.zeta, sd))));
     }
     return Pn;
 }
 
 template <typename C, typename KC, int MODE>
-__device__ inline C xf_element(const GridJob& J, long vi, const XfScalars& sc) {
+__device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc) {
     if (vi < 0) return mk<C>(0.0, 0.0);
+    const unsigned i = (unsigned)vi;
     if (MODE == XF_PLAIN) {
-        return static_cast<const C*>(J.xin)[vi];
+        return gload<C>(J.xin, i);
     } else if (MODE == XF_SOL) {
         if (sc.pending) {
-            const double2 pv = to_d(static_cast<const KC*>(J.p)[vi]);
-            return from_d<C>(apply_pending<KC>(J, vi, sc, pv, false));
+            const double2 pv = to_d(gload<KC>(J.p, i));
+            return from_d<C>(apply_pending<KC>(J, i, sc, pv, false));
         }
-        return from_d<C>(J.P[vi]);
+        return from_d<C>(gload<double2>(J.P, i));
     } else if (MODE == XF_P) {
-        KC* r = static_cast<KC*>(J.r);
-        KC* p = static_cast<KC*>(J.p);
-        const KC* v = static_cast<const KC*>(J.v);
-        const double2 pv = to_d(p[vi]);
+        const double2 pv = to_d(gload<KC>(J.p, i));
         double2 rv;
         if (sc.pending) {
-            const double2 td = to_d(static_cast<const KC*>(J.t)[vi]);
-            const double2 sd = to_d(static_cast<const KC*>(J.s)[vi]);
-            J.P[vi] = cadd(J.P[vi], cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
+            const double2 td = to_d(gload<KC>(J.t, i));
+            const double2 sd = to_d(gload<KC>(J.s, i));
+            gstore<double2>(J.P, i, cadd(gload<double2>(J.P, i), cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td))));
             const KC rk = from_d<KC>(csub(td, dcmul(sc.zeta, sd)));
-            r[vi] = rk;
+            gstore<KC>(J.r, i, rk);
             rv = to_d(rk);
         } else {
-            rv = to_d(r[vi]);
+            rv = to_d(gload<KC>(J.r, i));
         }
-        const double2 w = csub(pv, dcmul(sc.zeta, to_d(v[vi])));   // p - u, u = zeta v
+        const double2 w = csub(pv, dcmul(sc.zeta, to_d(gload<KC>(J.v, i))));   // p - u, u = zeta v
         const KC pk = from_d<KC>(cadd(rv, dcmul(sc.beta, w)));
-        p[vi] = pk;
+        gstore<KC>(J.p, i, pk);
         return from_d<C>(to_d(pk));
     } else {   // XF_T
-        const KC* r = static_cast<const KC*>(J.r);
-        const KC* v = static_cast<const KC*>(J.v);
-        KC* t = static_cast<KC*>(J.t);
-        const KC tk = from_d<KC>(csub(to_d(r[vi]), dcmul(sc.alpha, to_d(v[vi]))));
-        t[vi] = tk;
+        const KC tk = from_d<KC>(csub(to_d(gload<KC>(J.r, i)), dcmul(sc.alpha, to_d(gload<KC>(J.v, i)))));
+        gstore<KC>(J.t, i, tk);
         return from_d<C>(to_d(tk));
     }
 }
 
 // dipole vector index of box cell (row q = z*ny + y, x) component c, or -1
-__device__ inline long vec_index(const GridJob& J, int q, int x, int c) {
-    if (J.box_map == nullptr) return 3L * ((long)q * J.nx + x) + c;
-    const int dip = J.box_map[(long)q * J.nx + x];
-    return dip >= 0 ? 3L * dip + c : -1;
+__device__ inline int vec_index(const GridJob& J, int q, int x, int c) {
+    const int cell = q * J.nx + x;
+    if (J.box_map == nullptr) return 3 * cell + c;
+    const int dip = ((const TDDA_GLOBAL int*)J.box_map)[(unsigned)cell];
+    return dip >= 0 ? 3 * dip + c : -1;
 }
 
 // ----------------------------------------------------------------------------
@@ -236,29 +256,27 @@ __device__ inline long vec_index(const GridJob& J, int q, int x, int c) {
 constexpr int XI_NACC = 8;
 
 template <typename C, typename KC, int MODE>
-__device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __restrict__ al,
+__device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __restrict__ al,
                                   double (&acc)[XI_NACC]) {
-    const C a = al[vi / 3];
+    const unsigned i = (unsigned)vi;
+    const C a = gload<C>(al, i / 3u);
     if (MODE == XI_PLAIN) {
-        const C u = static_cast<const C*>(J.xin)[vi];
-        static_cast<C*>(J.yout)[vi] = cadd(conv, cmul(a, u));
+        const C u = gload<C>(J.xin, i);
+        gstore<C>(J.yout, i, cadd(conv, cmul(a, u)));
     } else if (MODE == XI_V) {
-        const KC* p = static_cast<const KC*>(J.p);
-        const KC* r0 = static_cast<const KC*>(J.r0);
-        const C y = cadd(conv, cmul(a, from_d<C>(to_d(p[vi]))));
+        const C y = cadd(conv, cmul(a, from_d<C>(to_d(gload<KC>(J.p, i)))));
         const KC vk = from_d<KC>(to_d(y));
-        static_cast<KC*>(J.v)[vi] = vk;
-        const double2 d = cmulc(to_d(vk), to_d(r0[vi]));   // conj(r0) v
+        gstore<KC>(J.v, i, vk);
+        const double2 d = cmulc(to_d(vk), to_d(gload<KC>(J.r0, i)));   // conj(r0) v
         acc[0] += d.x;
         acc[1] += d.y;
     } else if (MODE == XI_S) {
-        const KC* t = static_cast<const KC*>(J.t);
-        const KC tk = t[vi];
+        const KC tk = gload<KC>(J.t, i);
         const C y = cadd(conv, cmul(a, from_d<C>(to_d(tk))));
         const KC sk = from_d<KC>(to_d(y));
-        static_cast<KC*>(J.s)[vi] = sk;
+        gstore<KC>(J.s, i, sk);
         const double2 sd = to_d(sk), td = to_d(tk);
-        const double2 rz = to_d(static_cast<const KC*>(J.r0)[vi]);
+        const double2 rz = to_d(gload<KC>(J.r0, i));
         const double2 d = cmulc(td, sd);    // conj(s) t
         const double2 e = cmulc(td, rz);    // conj(r0) t
         const double2 f = cmulc(sd, rz);    // conj(r0) s
@@ -271,11 +289,11 @@ __device__ inline void xi_element(const GridJob& J, long vi, C conv, const C* __
         acc[6] += f.y;
         acc[7] += td.x * td.x + td.y * td.y;
     } else if (MODE == XI_RESID) {
-        const double2 y = cadd(to_d(conv), dcmul(to_d(a), J.P[vi]));
-        const double2 rr = csub(J.rhs[vi], y);
+        const double2 y = cadd(to_d(conv), dcmul(to_d(a), gload<double2>(J.P, i)));
+        const double2 rr = csub(gload<double2>(J.rhs, i), y);
         const KC rk = from_d<KC>(rr);
-        static_cast<KC*>(J.r)[vi] = rk;
-        const double2 d = cmulc(to_d(rk), to_d(static_cast<const KC*>(J.r0)[vi]));
+        gstore<KC>(J.r, i, rk);
+        const double2 d = cmulc(to_d(rk), to_d(gload<KC>(J.r0, i)));
         acc[0] += d.x;
         acc[1] += d.y;
         acc[2] += rr.x * rr.x + rr.y * rr.y;
@@ -331,26 +349,25 @@ __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC]) {
 // Ghat(L-k) = +-Ghat(k).
 // ----------------------------------------------------------------------------
 // The six components of one spectral point are one contiguous record.
-__device__ inline void green_load6(const float2* __restrict__ G, long off, float2 (&g)[6]) {
-    const float4* q = reinterpret_cast<const float4*>(G + 6 * off);
-    const float4 a = q[0], b = q[1], c = q[2];
+__device__ inline void green_load6(const float2* G, unsigned off, float2 (&g)[6]) {
+    const float4 a = gload<float4>(G, 3u * off), b = gload<float4>(G, 3u * off + 1u),
+                 c = gload<float4>(G, 3u * off + 2u);
     g[0] = make_float2(a.x, a.y); g[1] = make_float2(a.z, a.w);
     g[2] = make_float2(b.x, b.y); g[3] = make_float2(b.z, b.w);
     g[4] = make_float2(c.x, c.y); g[5] = make_float2(c.z, c.w);
 }
-__device__ inline void green_load6(const double2* __restrict__ G, long off, double2 (&g)[6]) {
+__device__ inline void green_load6(const double2* G, unsigned off, double2 (&g)[6]) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) g[k] = G[6 * off + k];
+    for (int k = 0; k < 6; ++k) g[k] = gload<double2>(G, 6u * off + k);
 }
 
 template <typename C>
-__device__ inline void green_apply(const C* __restrict__ G, const GridJob& J, int kx, int ky, int kz,
-                                   C& v0, C& v1, C& v2) {
+__device__ inline void green_apply(const C* G, const GridJob& J, int kx, int ky, int kz, C& v0, C& v1, C& v2) {
     int sx = 1, sy = 1, sz = 1;
     if (2 * kx > J.lx) { kx = J.lx - kx; sx = -1; }
     if (2 * ky > J.ly) { ky = J.ly - ky; sy = -1; }
     if (2 * kz > J.lz) { kz = J.lz - kz; sz = -1; }
-    const long off = ((long)kz * J.hy + ky) * J.hx + kx;
+    const unsigned off = ((unsigned)kz * J.hy + ky) * J.hx + kx;
     C g[6];
     green_load6(G, off, g);
     if (sx * sy < 0) g[1] = cscale(g[1], -1);

@@ -47,7 +47,7 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
         const int rem = e / 3;
         const int x = rem % nx;
         const int b = rem / nx;
-        const long vi = vec_index(J, q0 + b, x, c);
+        const int vi = vec_index(J, q0 + b, x, c);
         lds[x * bp + 3 * b + c] = xf_element<C, KC, MODE>(J, vi, sc);
     }
     __syncthreads();
@@ -113,7 +113,7 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
         const int rem = e / 3;
         const int x = rem % nx;
         const int b = rem / nx;
-        const long vi = vec_index(J, q0 + b, x, c);
+        const int vi = vec_index(J, q0 + b, x, c);
         if (vi < 0) continue;
         xi_element<C, KC, MODE>(J, vi, cscale(lds[x * bp + 3 * b + c], inv_m), al, acc);
     }
@@ -299,7 +299,7 @@ k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
 #pragma unroll
     for (int k = 0; k < UPD_EPT; ++k) {
         const long i = base + (long)k * FFT_THREADS + threadIdx.x;
-        if (i < n3) apply_pending<KC>(J, i, sc, to_d(p[i]), true);
+        if (i < n3) apply_pending<KC>(J, (unsigned)i, sc, to_d(p[i]), true);
     }
 }
 

@@ -12,7 +12,7 @@ namespace {
 
 template <typename C>
 __device__ inline void load_twiddles(C* twl, const C* __restrict__ twg, int L) {
-    for (int i = threadIdx.x; i < L; i += FFT_THREADS) twl[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += FFT_THREADS) twl[i] = gload<C>(twg, (unsigned)i);
 }
 
 // ----------------------------------------------------------------------------
@@ -44,15 +44,15 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
             });
         __syncthreads();
         C* g = static_cast<C*>(J.grid);
-        const int ny = J.ny, ly = J.ly;
-        const long S = (long)J.nz * ly * L;
+        const unsigned ny = J.ny, ly = J.ly;
+        const unsigned S = (unsigned)J.nz * ly * L;
         const unsigned long long mny = J.mny;
         for (int t = threadIdx.x; t < X::TB; t += FFT_THREADS)
             X::fwd_b(t, lds, nr, [&](int r, int c, int x, C v) {
                 const unsigned q = q0 + r;
                 const unsigned z = fdiv(q, mny);
                 const unsigned y = q - z * ny;
-                g[c * S + ((long)z * ly + y) * L + x] = v;
+                gstore<C>(g, c * S + (z * ly + y) * L + x, v);
             });
     });
 }
@@ -77,23 +77,23 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
         const int q0 = jt.y * fast_nr_x<Real>(L1, L2);
         const int nr = min(fast_nr_x<Real>(L1, L2), J.ny * J.nz - q0);
         const C* g = static_cast<const C*>(J.grid);
-        const int ny = J.ny, ly = J.ly;
-        const long S = (long)J.nz * ly * L;
+        const unsigned ny = J.ny, ly = J.ly;
+        const unsigned S = (unsigned)J.nz * ly * L;
         const unsigned long long mny = J.mny;
         __syncthreads();
         for (int t = threadIdx.x; t < X::TA; t += FFT_THREADS)
-            X::inv_a(t, lds, twl, nr, [&](int r, int c, int x) {
+            X::inv_a(t, lds, twl, nr, [&](int r, int c, int x) -> C {
                 const unsigned q = q0 + r;
                 const unsigned z = fdiv(q, mny);
                 const unsigned y = q - z * ny;
-                return g[c * S + ((long)z * ly + y) * L + x];
+                return gload<C>(g, c * S + (z * ly + y) * L + x);
             });
         __syncthreads();
         const Real inv_m = (Real)(1.0 / ((double)J.lx * (double)J.ly * (double)J.lz));
         const C* al = Tile<Real>::al(J);
         for (int t = threadIdx.x; t < X::TB; t += FFT_THREADS)
             X::inv_b(t, lds, nr, J.nx, [&](int r, int c, int x, C v) {
-                const long vi = vec_index(J, q0 + r, x, c);
+                const int vi = vec_index(J, q0 + r, x, c);
                 if (vi >= 0) xi_element<C, KC, MODE>(J, vi, cscale(v, inv_m), al, acc);
             });
     });
@@ -119,24 +119,24 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
         C* lds = reinterpret_cast<C*>(smem_yf);
         C* twl = lds + Y::LDS;
         load_twiddles(twl, twpool + J.py.tw_off, L);
-        const int lx = J.lx;
+        const unsigned lx = J.lx;
         const int pl0 = jt.y * NP;
-        const int nb = min(NP, 3 * J.nz * lx - pl0);
+        const int nb = min(NP, 3 * J.nz * (int)lx - pl0);
         const unsigned long long mlx = J.mlx;
         C* g = static_cast<C*>(J.grid);
-        auto base = [&](int b) -> long {
+        auto base = [&](int b) -> unsigned {
             const unsigned p = pl0 + b;
             const unsigned cz = fdiv(p, mlx);
-            return (long)cz * L * lx + (p - cz * lx);
+            return cz * L * lx + (p - cz * lx);
         };
         const int n_in = DIR < 0 ? J.ny : L;
         const int n_out = DIR < 0 ? L : J.ny;
         __syncthreads();
         for (int t = threadIdx.x; t < Y::TA; t += FFT_THREADS)
-            Y::step_a(t, lds, twl, nb, n_in, [&](int b, int pos) { return g[base(b) + (long)pos * lx]; });
+            Y::step_a(t, lds, twl, nb, n_in, [&](int b, int pos) -> C { return gload<C>(g, base(b) + (unsigned)pos * lx); });
         __syncthreads();
         for (int t = threadIdx.x; t < Y::TB; t += FFT_THREADS)
-            Y::step_b(t, lds, nb, n_out, [&](int b, int pos, C v) { g[base(b) + (long)pos * lx] = v; });
+            Y::step_b(t, lds, nb, n_out, [&](int b, int pos, C v) { gstore<C>(g, base(b) + (unsigned)pos * lx, v); });
     });
 }
 
@@ -159,17 +159,18 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
         C* lds = reinterpret_cast<C*>(smem_zf);
         C* twl = lds + Z::LDS;
         load_twiddles(twl, twpool + J.pz.tw_off, L);
-        const int lx = J.lx, nz = J.nz;
-        const long plane = (long)J.ly * lx;
-        const long S = (long)nz * plane;
+        const unsigned lx = J.lx;
+        const int nz = J.nz;
+        const unsigned plane = (unsigned)J.ly * lx;
+        const unsigned S = (unsigned)nz * plane;
         const int pl0 = jt.y * NP;
-        const int nb = min(NP, J.ly * lx - pl0);
+        const int nb = min(NP, (int)plane - pl0);
         C* g = static_cast<C*>(J.grid) + pl0;
         const C* G = Tile<Real>::green(J);
         const unsigned long long mlx = J.mlx;
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TA; t += FFT_THREADS)
-            Z::fwd_a(t, lds, twl, nb, nz, [&](int c, int b, int pos) { return g[c * S + pos * plane + b]; });
+            Z::fwd_a(t, lds, twl, nb, nz, [&](int c, int b, int pos) -> C { return gload<C>(g, c * S + pos * plane + b); });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TB; t += FFT_THREADS)
             Z::mid_b(t, lds, twl, nb, [&](int b, int kz, C& v0, C& v1, C& v2) {
@@ -179,7 +180,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
             });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TA; t += FFT_THREADS)
-            Z::inv_a(t, lds, nb, nz, [&](int c, int b, int pos, C v) { g[c * S + pos * plane + b] = v; });
+            Z::inv_a(t, lds, nb, nz, [&](int c, int b, int pos, C v) { gstore<C>(g, c * S + pos * plane + b, v); });
     });
 }
 
