@@ -49,12 +49,22 @@ template <> struct Tile<double> {
     __device__ static const double2* al(const GridJob& J) { return J.al64; }
 };
 
+// GATE_ACTIVE: a running job not waiting for its reliable update;
+// GATE_RU: a running job waiting for it.
 __device__ inline bool gate_open(const GridJob& J, int gate) {
     if (gate == GATE_ALWAYS) return true;
     const KScalars* ks = J.ks;
     if (ks->status != JOB_ACTIVE) return false;
     if (gate == GATE_RU) return ks->need_ru != 0;
-    return true;
+    return ks->need_ru == 0;
+}
+
+// True when a reliable-update launch has nothing to do: no job of any batch
+// waits for one (ru_jobs counts them; one load instead of the per-block
+// map -> job -> scalars chain).  The per-job gate still decides otherwise.
+__device__ inline bool ru_idle(int gate, const int* ru_jobs) {
+    return gate == GATE_RU && ru_jobs != nullptr &&
+           __hip_atomic_load(ru_jobs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
 }
 
 // Deterministic block reduction of NV doubles; every thread gets the sums.
@@ -318,7 +328,7 @@ __device__ inline void xi_finish(const GridJob& J, int tile, double (&acc)[XI_NA
 //   XI_RESID : true residual of the updated P (the X forward pass applied the
 //              pending update): reliable-update bookkeeping and decision
 template <int MODE>
-__device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC]) {
+__device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int* ru_jobs) {
     const double2 dot = make_double2(sum[0], sum[1]);
     if (MODE == XI_V) {
         ks->alpha = ddiv(ks->rho, dot);
@@ -333,7 +343,9 @@ __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC]) {
         const double rn2 = fmax(sum[7] - (dot.x * dot.x + dot.y * dot.y) / ss, 0.0);
         ks->pending = 1;
         step_decide(ks, rho_new, rn2);
+        if (ks->need_ru != 0 && ru_jobs != nullptr) atomicAdd(ru_jobs, 1);
     } else if (MODE == XI_RESID) {
+        if (ks->need_ru != 0 && ru_jobs != nullptr) atomicSub(ru_jobs, 1);
         ks->pending = 0;
         ks->need_ru = 0;
         ks->n_ru += 1;

@@ -43,7 +43,10 @@ enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
 struct LaunchSet {
     const GridJob* jobs = nullptr;   // device
     int njobs = 0;
-    PassMap pm[2][3][2];             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
+    PassMap pm[2][3][2];
+    // device count of jobs (of any batch) that wait for a reliable update;
+    // launches gated on them return at once while it is zero
+    int* ru_jobs = nullptr;             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
     const int2* map_upd = nullptr;
     int n_upd = 0;
     bool refined = false;            // Krylov vectors are c64
@@ -70,6 +73,9 @@ void launch_finalize(const LaunchSet& ls, hipStream_t st);
 // Initial residual, r0, p = v = 0 and scalars; warm: r = rhs - yout.
 void launch_init(const LaunchSet& ls, bool warm, hipStream_t st);
 
+// The device counter behind LaunchSet::ru_jobs (one per process, zeroed).
+int* ru_job_counter();
+
 // Host-side tile geometry for a job (fills the b*/t*/f* fields).
 void plan_tiles(GridJob& J);
 // Dynamic LDS bytes of the job's tile in a pass (generic or specialised).
@@ -77,13 +83,13 @@ size_t lds_bytes(const GridJob& J, int pass, bool c64);
 
 // Length-specialised kernels (passes_fast.hip).
 void launch_fast_xfwd(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, hipStream_t st);
+                      int mode, int gate, const int* ru, hipStream_t st);
 void launch_fast_xinv(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, hipStream_t st);
+                      int mode, int gate, const int* ru, hipStream_t st);
 void launch_fast_y(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int dir, int gate,
-                   hipStream_t st);
+                   const int* ru, hipStream_t st);
 void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int gate,
-                   hipStream_t st);
+                   const int* ru, hipStream_t st);
 // Rows (X) or pencils (Y, Z) per tile and LDS bytes of the specialised path
 // for length L; false if L has no specialised split.
 bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes);

@@ -24,12 +24,13 @@ namespace tdda {
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
+k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ ru_jobs) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_x[];
     C* lds = reinterpret_cast<C*>(smem_x);
+    if (ru_idle(gate, ru_jobs)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -72,12 +73,13 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate) {
+k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ ru_jobs) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_xi[];
     C* lds = reinterpret_cast<C*>(smem_xi);
+    if (ru_idle(gate, ru_jobs)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -125,11 +127,12 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
 // ----------------------------------------------------------------------------
 template <typename Real, int DIR>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate) {
+k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ ru_jobs) {
     using C = cx_t<Real>;
     constexpr int CAP = Caps<Real>::Y;
     extern __shared__ __align__(16) unsigned char smem_y[];
     C* lds = reinterpret_cast<C*>(smem_y);
+    if (ru_idle(gate, ru_jobs)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -215,11 +218,12 @@ __device__ inline void z_bridge_rt(int R, int np, int L, const TW* __restrict__ 
 
 template <typename Real>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate) {
+k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ ru_jobs) {
     using C = cx_t<Real>;
     constexpr int CAP = Caps<Real>::Z;
     extern __shared__ __align__(16) unsigned char smem_z[];
     C* lds = reinterpret_cast<C*>(smem_z);
+    if (ru_idle(gate, ru_jobs)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -317,12 +321,13 @@ __global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
 // ----------------------------------------------------------------------------
 template <typename Real, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_xi_reduce(const GridJob* __restrict__ jobs, int gate) {
+k_xi_reduce(const GridJob* __restrict__ jobs, int gate, int* __restrict__ ru_jobs) {
+    if (ru_idle(gate, ru_jobs)) return;
     const GridJob& J = jobs[blockIdx.x];
     if (!gate_open(J, gate)) return;
     double sum[XI_NACC];
     job_sum<XI_NACC>(J, Tile<Real>::tx(J), sum);
-    if (threadIdx.x == 0) xi_decide<MODE>(J.ks, sum);
+    if (threadIdx.x == 0) xi_decide<MODE>(J.ks, sum, ru_jobs);
 }
 
 // ----------------------------------------------------------------------------
@@ -389,46 +394,47 @@ k_init_reduce(const GridJob* __restrict__ jobs, int warm) {
 // host launchers
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal>
-static void xfwd_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xf, int gate,
+static void xfwd_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xf, int gate, const int* ru,
                          hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xf) {
-        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
         default: throw Error(ERR_INVALID, "bad X-forward mode");
     }
 }
 
 template <typename Real, typename KReal>
-static void xinv_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xi, int gate,
+static void xinv_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xi, int gate, const int* ru,
                          hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xi) {
-        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
-        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate); break;
+        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
         default: throw Error(ERR_INVALID, "bad X-inverse mode");
     }
 }
 
 template <typename Real>
-static void ypass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int dir, int gate,
+static void ypass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int dir, int gate, const int* ru,
                           hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
-    if (dir < 0) hipLaunchKernelGGL((k_ypass<Real, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate);
-    else hipLaunchKernelGGL((k_ypass<Real, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate);
+    if (dir < 0) hipLaunchKernelGGL((k_ypass<Real, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru);
+    else hipLaunchKernelGGL((k_ypass<Real, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru);
 }
 
 template <typename Real>
-static void zpass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int gate, hipStream_t st) {
+static void zpass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int gate, const int* ru,
+                          hipStream_t st) {
     if (pm.n == 0) return;
-    hipLaunchKernelGGL((k_zpass<Real>), dim3(pm.n), dim3(FFT_THREADS), pm.lds, st, jobs, pm.map, tw, gate);
+    hipLaunchKernelGGL((k_zpass<Real>), dim3(pm.n), dim3(FFT_THREADS), pm.lds, st, jobs, pm.map, tw, gate, ru);
 }
 
 // One pass of the matvec (0 X fwd, 1 Y fwd, 2 Z, 3 Y inv, 4 X inv): the
@@ -441,27 +447,27 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
     const cx_t<Real>* t = c64 ? reinterpret_cast<const cx_t<Real>*>(tw.tw32) : reinterpret_cast<const cx_t<Real>*>(tw.tw64);
     switch (k) {
         case 0:
-            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, st);
-            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, st);
+            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, ls.ru_jobs, st);
+            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, ls.ru_jobs, st);
             break;
         case 1:
         case 3:
-            ypass_generic<Real>(ls.pm[pr][PASS_Y][0], ls.jobs, t, k == 1 ? -1 : 1, gate, st);
-            launch_fast_y(ls.pm[pr][PASS_Y][1], ls.jobs, tw, c64, k == 1 ? -1 : 1, gate, st);
+            ypass_generic<Real>(ls.pm[pr][PASS_Y][0], ls.jobs, t, k == 1 ? -1 : 1, gate, ls.ru_jobs, st);
+            launch_fast_y(ls.pm[pr][PASS_Y][1], ls.jobs, tw, c64, k == 1 ? -1 : 1, gate, ls.ru_jobs, st);
             break;
         case 2:
-            zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, st);
-            launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, st);
+            zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, ls.ru_jobs, st);
+            launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, ls.ru_jobs, st);
             break;
         default:
-            xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
-            launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, st);
+            xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, ls.ru_jobs, st);
+            launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, ls.ru_jobs, st);
             if (xi != XI_PLAIN && ls.njobs > 0) {
                 const dim3 grid(ls.njobs), blk(FFT_THREADS);
                 switch (xi) {
-                    case XI_V: hipLaunchKernelGGL((k_xi_reduce<Real, XI_V>), grid, blk, 0, st, ls.jobs, gate); break;
-                    case XI_S: hipLaunchKernelGGL((k_xi_reduce<Real, XI_S>), grid, blk, 0, st, ls.jobs, gate); break;
-                    case XI_RESID: hipLaunchKernelGGL((k_xi_reduce<Real, XI_RESID>), grid, blk, 0, st, ls.jobs, gate); break;
+                    case XI_V: hipLaunchKernelGGL((k_xi_reduce<Real, XI_V>), grid, blk, 0, st, ls.jobs, gate, ls.ru_jobs); break;
+                    case XI_S: hipLaunchKernelGGL((k_xi_reduce<Real, XI_S>), grid, blk, 0, st, ls.jobs, gate, ls.ru_jobs); break;
+                    case XI_RESID: hipLaunchKernelGGL((k_xi_reduce<Real, XI_RESID>), grid, blk, 0, st, ls.jobs, gate, ls.ru_jobs); break;
                     default: throw Error(ERR_INVALID, "bad X-inverse mode");
                 }
             }

@@ -427,6 +427,17 @@ void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool re
     build(systems, ks, refined, st, true);
 }
 
+int* ru_job_counter() {
+    static std::mutex mu;
+    static int* ptr = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (ptr == nullptr) {
+        TDDA_HIP(hipMalloc(&ptr, sizeof(int)));
+        TDDA_HIP(hipMemset(ptr, 0, sizeof(int)));
+    }
+    return ptr;
+}
+
 namespace {
 
 // Z tiles of one job ordered so that tiles whose pencils are x/y mirrors of
@@ -543,6 +554,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     ls = LaunchSet{};
     ls.jobs = djobs_.get();
     ls.njobs = (int)hjobs.size();
+    ls.ru_jobs = ru_job_counter();
     for (int pr = 0; pr < 2; ++pr)
         for (int p = 0; p < 3; ++p)
             for (int k = 0; k < 2; ++k) {

@@ -13,7 +13,14 @@ Server::Server(const ServerOptions& o, const SiciDevice& tables) : opt_(o), tabl
     TDDA_HIP(hipStreamCreate(&main_));
     TDDA_HIP(hipStreamCreate(&setup_));
     ks_.alloc(o.max_slots);
-    ks_host_.assign(o.max_slots, KScalars{});
+    ks_.zero(main_);
+    for (int i = 0; i < 2; ++i) {
+        TDDA_HIP(hipHostMalloc((void**)&snap_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
+        TDDA_HIP(hipHostMalloc((void**)&init_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
+        std::memset(snap_[i], 0, o.max_slots * sizeof(KScalars));
+        TDDA_HIP(hipEventCreateWithFlags(&chunk_done_[i], hipEventDisableTiming));
+    }
+    TDDA_HIP(hipStreamSynchronize(main_));
     slots_.resize(o.max_slots);
     th_ = std::thread([this] { loop(); });
 }
@@ -29,6 +36,11 @@ Server::~Server() {
     (void)hipStreamSynchronize(setup_);
     slots_.clear();
     pool_.clear();
+    for (int i = 0; i < 2; ++i) {
+        if (snap_[i]) (void)hipHostFree(snap_[i]);
+        if (init_[i]) (void)hipHostFree(init_[i]);
+        if (chunk_done_[i]) (void)hipEventDestroy(chunk_done_[i]);
+    }
     (void)hipStreamDestroy(main_);
     (void)hipStreamDestroy(setup_);
 }
@@ -112,6 +124,7 @@ std::vector<int> Server::admit() {
             joined.push_back(free_slot);
         }
         Slot& sl = slots_[joined.back()];
+        sl.gen++;
         const auto key = std::make_tuple(req.desc.nx, req.desc.ny, req.desc.nz);
         std::unique_ptr<System> reuse;
         auto pit = pool_.find(key);
@@ -131,7 +144,7 @@ std::vector<int> Server::admit() {
     return joined;
 }
 
-void Server::retire(const std::vector<int>& fin) {
+void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
     if (fin.empty()) return;
     std::vector<DevBuf<double2>> ebuf(fin.size());
     std::vector<std::vector<double2>> eh(fin.size());
@@ -149,7 +162,7 @@ void Server::retire(const std::vector<int>& fin) {
     std::lock_guard<std::mutex> lk(mu_);
     for (size_t k = 0; k < fin.size(); ++k) {
         Slot& sl = slots_[fin[k]];
-        const KScalars& ks = ks_host_[fin[k]];
+        const KScalars& ks = snap[fin[k]];
         ServeResult r;
         r.id = sl.req.id;
         r.status = ks.status;
@@ -172,86 +185,94 @@ void Server::retire(const std::vector<int>& fin) {
     }
 }
 
-void Server::rebuild() {
-    std::vector<System*> sys;
-    std::vector<KScalars*> ks;
-    for (int s : active_) {
-        sys.push_back(slots_[s].prob->sys.get());
-        ks.push_back(ks_.get() + s);
-    }
-    if (!sys.empty()) batch_.build(sys, ks, opt_.refined, main_);
-    std::lock_guard<std::mutex> lk(mu_);
-    stats_.rebuilds++;
-    stats_.max_active = std::max(stats_.max_active, (int)active_.size());
-}
-
 void Server::loop() {
     try {
-        TwiddlePools tw = PlanRegistry::instance().pools();
         hipEvent_t setup_done;
         TDDA_HIP(hipEventCreateWithFlags(&setup_done, hipEventDisableTiming));
+        int cur = 0;   // batch whose chunk is enqueued next
         while (true) {
-            bool ran = false;
-            if (!active_.empty()) {
-                tw = PlanRegistry::instance().pools();
+            // 1. enqueue a chunk of iterations for batch `cur`
+            if (!members_[cur].empty()) {
+                const TwiddlePools tw = PlanRegistry::instance().pools();
+                const LaunchSet& ls = batch_[cur].ls;
                 for (int k = 0; k < opt_.chunk; ++k) {
-                    launch_matvec(batch_.ls, tw, opt_.refined, XF_P, XI_V, GATE_ACTIVE, main_);
-                    launch_matvec(batch_.ls, tw, opt_.refined, XF_T, XI_S, GATE_ACTIVE, main_);
-                    if (opt_.refined) launch_matvec(batch_.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, main_);
+                    launch_matvec(ls, tw, opt_.refined, XF_P, XI_V, GATE_ACTIVE, main_);
+                    launch_matvec(ls, tw, opt_.refined, XF_T, XI_S, GATE_ACTIVE, main_);
+                    if (opt_.refined) launch_matvec(ls, tw, false, XF_SOL, XI_RESID, GATE_RU, main_);
                 }
-                launch_finalize(batch_.ls, main_);
-                TDDA_HIP(hipMemcpyAsync(ks_host_.data(), ks_.get(), ks_host_.size() * sizeof(KScalars),
+                launch_finalize(ls, main_);
+                TDDA_HIP(hipMemcpyAsync(snap_[cur], ks_.get(), opt_.max_slots * sizeof(KScalars),
                                         hipMemcpyDeviceToHost, main_));
-                ran = true;
+                TDDA_HIP(hipEventRecord(chunk_done_[cur], main_));
+                inflight_[cur] = true;
+                std::lock_guard<std::mutex> lk(mu_);
+                stats_.chunks++;
+                stats_.launches += (long)opt_.chunk * (opt_.refined ? 18 : 12) + 3;
             }
-            // set up arrivals while the batch runs
+            // 2. admit arrivals (operators and densities on the setup stream)
             std::vector<int> joined = admit();
             if (!joined.empty()) TDDA_HIP(hipEventRecord(setup_done, setup_));
-            if (ran) TDDA_HIP(hipStreamSynchronize(main_));
+            // 3. statuses of the previous chunk (the current one keeps running)
+            const int prev = 1 - cur;
+            // (a slot retired after an earlier chunk may already hold a new
+            // request: only the generation that ran in that chunk counts)
+            std::vector<int> fin;
+            if (inflight_[prev]) {
+                TDDA_HIP(hipEventSynchronize(chunk_done_[prev]));
+                inflight_[prev] = false;
+                for (const auto& m : members_[prev])
+                    if (slots_[m.first].used && slots_[m.first].gen == m.second &&
+                        snap_[prev][m.first].status != JOB_ACTIVE)
+                        fin.push_back(m.first);
+            }
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                if (ran) {
-                    stats_.chunks++;
-                    stats_.launches += (long)opt_.chunk * (opt_.refined ? 15 : 10) + 1;
-                }
                 stats_.peak_bytes = std::max(stats_.peak_bytes, device_bytes_in_use());
             }
-            bool changed = false;
-            if (ran) {
-                std::vector<int> fin, keep;
-                for (int s : active_) (ks_host_[s].status != JOB_ACTIVE ? fin : keep).push_back(s);
-                if (!fin.empty()) {
-                    retire(fin);
-                    active_ = keep;
-                    changed = true;
-                }
-            }
+            // jobs still in the running batch are gated off once finished,
+            // so their slots and buffers can be recycled now
+            retire(fin, snap_[prev]);
+            // 4. build the other batch for the next chunk
+            std::vector<std::pair<int, unsigned>> next;
+            for (const auto& m : members_[cur])
+                if (slots_[m.first].used && slots_[m.first].gen == m.second) next.push_back(m);
             if (!joined.empty()) {
                 TDDA_HIP(hipStreamWaitEvent(main_, setup_done, 0));
-                std::vector<KScalars> init(joined.size());
                 std::vector<System*> js;
                 std::vector<KScalars*> jk;
                 for (size_t k = 0; k < joined.size(); ++k) {
-                    std::memset(&init[k], 0, sizeof(KScalars));
-                    init[k].tol = opt_.tol;
-                    init[k].maxit = opt_.maxit;
-                    init[k].refined = opt_.refined ? 1 : 0;
-                    init[k].ru_delta = opt_.ru_delta;
-                    init[k].status = JOB_ACTIVE;
-                    TDDA_HIP(hipMemcpyAsync(ks_.get() + joined[k], &init[k], sizeof(KScalars),
-                                            hipMemcpyHostToDevice, main_));
+                    KScalars& in = init_[prev][k];
+                    std::memset(&in, 0, sizeof(KScalars));
+                    in.tol = opt_.tol;
+                    in.maxit = opt_.maxit;
+                    in.refined = opt_.refined ? 1 : 0;
+                    in.ru_delta = opt_.ru_delta;
+                    in.status = JOB_ACTIVE;
+                    TDDA_HIP(hipMemcpyAsync(ks_.get() + joined[k], &in, sizeof(KScalars), hipMemcpyHostToDevice,
+                                            main_));
                     js.push_back(slots_[joined[k]].prob->sys.get());
                     jk.push_back(ks_.get() + joined[k]);
                 }
-                Batch ib;
-                ib.build(js, jk, opt_.refined, main_, false);
-                launch_init(ib.ls, false, main_);
-                TDDA_HIP(hipStreamSynchronize(main_));   // ib's buffers die here
-                for (int s : joined) active_.push_back(s);
-                changed = true;
+                init_batch_[prev].build(js, jk, opt_.refined, main_, false);
+                launch_init(init_batch_[prev].ls, false, main_);
+                for (int s : joined) next.emplace_back(s, slots_[s].gen);
             }
-            if (changed) rebuild();
-            if (active_.empty()) {
+            members_[prev] = next;
+            if (!next.empty()) {
+                std::vector<System*> sys;
+                std::vector<KScalars*> ks;
+                for (const auto& m : next) {
+                    sys.push_back(slots_[m.first].prob->sys.get());
+                    ks.push_back(ks_.get() + m.first);
+                }
+                batch_[prev].build(sys, ks, opt_.refined, main_);
+                std::lock_guard<std::mutex> lk(mu_);
+                stats_.rebuilds++;
+                stats_.max_active = std::max(stats_.max_active, (int)next.size());
+            }
+            cur = prev;
+            // 5. idle: nothing running, queued or in flight
+            if (members_[cur].empty() && !inflight_[0] && !inflight_[1]) {
                 std::unique_lock<std::mutex> lk(mu_);
                 if (stop_) break;
                 if (pending_.empty())
@@ -262,6 +283,7 @@ void Server::loop() {
                 if (stop_) break;
             }
         }
+        (void)hipStreamSynchronize(main_);
         (void)hipEventDestroy(setup_done);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);

@@ -70,6 +70,7 @@ class Server {
    private:
     struct Slot {
         bool used = false;
+        unsigned gen = 0;   // bumped at every admission into the slot
         std::unique_ptr<Problem> prob;
         ServeRequest req;
         double t_admit = 0.0;
@@ -77,8 +78,7 @@ class Server {
     };
     void loop();
     std::vector<int> admit();
-    void retire(const std::vector<int>& slots);
-    void rebuild();
+    void retire(const std::vector<int>& slots, const KScalars* snap);
 
     ServerOptions opt_;
     SiciDevice tables_;
@@ -93,14 +93,19 @@ class Server {
     std::string error_;
     ServerStats stats_;
 
-    // loop-thread state
+    // loop-thread state.  Two batches alternate: while the chunk of one runs
+    // on the GPU, the host retires what the previous chunk finished, admits
+    // arrivals and builds the other batch for the next chunk.
     hipStream_t main_ = nullptr, setup_ = nullptr;
     DevBuf<KScalars> ks_;
-    std::vector<KScalars> ks_host_;
+    KScalars* snap_[2] = {nullptr, nullptr};   // pinned copies of ks_ after each chunk
+    KScalars* init_[2] = {nullptr, nullptr};   // pinned initial scalars of joining slots
+    hipEvent_t chunk_done_[2] = {nullptr, nullptr};   // chunk of batch i complete (snap_[i] valid)
+    bool inflight_[2] = {false, false};
     std::vector<Slot> slots_;
-    std::vector<int> active_;
+    std::vector<std::pair<int, unsigned>> members_[2];   // (slot, generation) in batch i
+    Batch batch_[2], init_batch_[2];
     std::map<std::tuple<int, int, int>, std::vector<std::unique_ptr<System>>> pool_;
-    Batch batch_;
     int busy_ = 0;   // requests admitted but not finished (guarded by mu_)
 };
 
