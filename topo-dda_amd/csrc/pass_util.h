@@ -59,6 +59,11 @@ __device__ inline bool gate_open(const GridJob& J, int gate) {
     return ks->need_ru == 0;
 }
 
+// Blocks of a compacted launch beyond the filled length return at once.
+__device__ inline bool beyond_limit(const int* limit) {
+    return limit != nullptr && (int)blockIdx.x >= __hip_atomic_load(limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // True when a reliable-update launch has nothing to do: no job of any batch
 // waits for one (ru_jobs counts them; one load instead of the per-block
 // map -> job -> scalars chain).  The per-job gate still decides otherwise.

@@ -33,6 +33,7 @@ struct PassMap {
     const int2* map = nullptr;
     int n = 0;
     size_t lds = 0;
+    const int* limit = nullptr;   // device count of valid map entries (blocks beyond it return)
 };
 enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
 
@@ -44,9 +45,16 @@ struct LaunchSet {
     const GridJob* jobs = nullptr;   // device
     int njobs = 0;
     PassMap pm[2][3][2];
-    // device count of jobs (of any batch) that wait for a reliable update;
-    // launches gated on them return at once while it is zero
-    int* ru_jobs = nullptr;             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
+    // device count of jobs (of any batch) that wait for a reliable update
+    int* ru_jobs = nullptr;
+    // reliable-update launches run over compacted copies of the c128 maps
+    // (only tiles of waiting jobs): ru_map holds the six regions in the order
+    // of pm[1] (X gen, X spec, Y gen, Y spec, Z gen, Z spec) at ru_off[k],
+    // ru_count[k] their filled lengths
+    int2* ru_map = nullptr;
+    int* ru_count = nullptr;
+    const int2* ru_src = nullptr;    // the c128 maps, contiguous, same regions
+    int ru_off[7] = {0, 0, 0, 0, 0, 0, 0};             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
     const int2* map_upd = nullptr;
     int n_upd = 0;
     bool refined = false;            // Krylov vectors are c64
@@ -83,13 +91,13 @@ size_t lds_bytes(const GridJob& J, int pass, bool c64);
 
 // Length-specialised kernels (passes_fast.hip).
 void launch_fast_xfwd(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, const int* ru, hipStream_t st);
+                      int mode, int gate, hipStream_t st);
 void launch_fast_xinv(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, const int* ru, hipStream_t st);
+                      int mode, int gate, hipStream_t st);
 void launch_fast_y(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int dir, int gate,
-                   const int* ru, hipStream_t st);
+                   hipStream_t st);
 void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int gate,
-                   const int* ru, hipStream_t st);
+                   hipStream_t st);
 // Rows (X) or pencils (Y, Z) per tile and LDS bytes of the specialised path
 // for length L; false if L has no specialised split.
 bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes);

@@ -24,13 +24,13 @@ namespace tdda {
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ ru_jobs) {
+k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_x[];
     C* lds = reinterpret_cast<C*>(smem_x);
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -73,13 +73,13 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ ru_jobs) {
+k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ tw, int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     constexpr int CAP = Caps<Real>::X;
     extern __shared__ __align__(16) unsigned char smem_xi[];
     C* lds = reinterpret_cast<C*>(smem_xi);
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -127,12 +127,12 @@ k_xinv(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
 // ----------------------------------------------------------------------------
 template <typename Real, int DIR>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ ru_jobs) {
+k_ypass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     constexpr int CAP = Caps<Real>::Y;
     extern __shared__ __align__(16) unsigned char smem_y[];
     C* lds = reinterpret_cast<C*>(smem_y);
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -218,12 +218,12 @@ __device__ inline void z_bridge_rt(int R, int np, int L, const TW* __restrict__ 
 
 template <typename Real>
 __global__ void __launch_bounds__(FFT_THREADS)
-k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ ru_jobs) {
+k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool, int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     constexpr int CAP = Caps<Real>::Z;
     extern __shared__ __align__(16) unsigned char smem_z[];
     C* lds = reinterpret_cast<C*>(smem_z);
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -316,6 +316,27 @@ __global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
 }
 
 // ----------------------------------------------------------------------------
+// compaction of the c128 pass maps to the tiles of jobs waiting for a
+// reliable update (unordered within a region; counts zeroed by the caller)
+// ----------------------------------------------------------------------------
+struct RuRegions {
+    int off[7];
+};
+__global__ void k_ru_compact(const GridJob* __restrict__ jobs, const int2* __restrict__ src, RuRegions rg,
+                             int2* __restrict__ dst, int* __restrict__ count, const int* __restrict__ ru_jobs) {
+    if (__hip_atomic_load(ru_jobs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rg.off[6]) return;
+    const int2 e = src[i];
+    const KScalars* ks = jobs[e.x].ks;
+    if (ks->status != JOB_ACTIVE || ks->need_ru == 0) return;
+    int k = 0;
+    while (k < 5 && i >= rg.off[k + 1]) ++k;
+    const int slot = atomicAdd(&count[k], 1);
+    dst[rg.off[k] + slot] = e;
+}
+
+// ----------------------------------------------------------------------------
 // per-job sums of the X-inverse dot partials and the scalar step (one block
 // per job, after the X inverse launches of a matvec)
 // ----------------------------------------------------------------------------
@@ -394,47 +415,47 @@ k_init_reduce(const GridJob* __restrict__ jobs, int warm) {
 // host launchers
 // ----------------------------------------------------------------------------
 template <typename Real, typename KReal>
-static void xfwd_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xf, int gate, const int* ru,
+static void xfwd_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xf, int gate,
                          hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xf) {
-        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_P: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_T: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_SOL: hipLaunchKernelGGL((k_xfwd<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
         default: throw Error(ERR_INVALID, "bad X-forward mode");
     }
 }
 
 template <typename Real, typename KReal>
-static void xinv_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xi, int gate, const int* ru,
+static void xinv_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int xi, int gate,
                          hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (xi) {
-        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XI_PLAIN: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_V: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_S: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_RESID: hipLaunchKernelGGL((k_xinv<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
         default: throw Error(ERR_INVALID, "bad X-inverse mode");
     }
 }
 
 template <typename Real>
-static void ypass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int dir, int gate, const int* ru,
+static void ypass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int dir, int gate,
                           hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
-    if (dir < 0) hipLaunchKernelGGL((k_ypass<Real, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru);
-    else hipLaunchKernelGGL((k_ypass<Real, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru);
+    if (dir < 0) hipLaunchKernelGGL((k_ypass<Real, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit);
+    else hipLaunchKernelGGL((k_ypass<Real, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit);
 }
 
 template <typename Real>
-static void zpass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int gate, const int* ru,
+static void zpass_generic(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int gate,
                           hipStream_t st) {
     if (pm.n == 0) return;
-    hipLaunchKernelGGL((k_zpass<Real>), dim3(pm.n), dim3(FFT_THREADS), pm.lds, st, jobs, pm.map, tw, gate, ru);
+    hipLaunchKernelGGL((k_zpass<Real>), dim3(pm.n), dim3(FFT_THREADS), pm.lds, st, jobs, pm.map, tw, gate, pm.limit);
 }
 
 // One pass of the matvec (0 X fwd, 1 Y fwd, 2 Z, 3 Y inv, 4 X inv): the
@@ -447,21 +468,21 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
     const cx_t<Real>* t = c64 ? reinterpret_cast<const cx_t<Real>*>(tw.tw32) : reinterpret_cast<const cx_t<Real>*>(tw.tw64);
     switch (k) {
         case 0:
-            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, ls.ru_jobs, st);
-            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, ls.ru_jobs, st);
+            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, st);
+            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, st);
             break;
         case 1:
         case 3:
-            ypass_generic<Real>(ls.pm[pr][PASS_Y][0], ls.jobs, t, k == 1 ? -1 : 1, gate, ls.ru_jobs, st);
-            launch_fast_y(ls.pm[pr][PASS_Y][1], ls.jobs, tw, c64, k == 1 ? -1 : 1, gate, ls.ru_jobs, st);
+            ypass_generic<Real>(ls.pm[pr][PASS_Y][0], ls.jobs, t, k == 1 ? -1 : 1, gate, st);
+            launch_fast_y(ls.pm[pr][PASS_Y][1], ls.jobs, tw, c64, k == 1 ? -1 : 1, gate, st);
             break;
         case 2:
-            zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, ls.ru_jobs, st);
-            launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, ls.ru_jobs, st);
+            zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, st);
+            launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, st);
             break;
         default:
-            xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, ls.ru_jobs, st);
-            launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, ls.ru_jobs, st);
+            xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
+            launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, st);
             if (xi != XI_PLAIN && ls.njobs > 0) {
                 const dim3 grid(ls.njobs), blk(FFT_THREADS);
                 switch (xi) {
@@ -479,7 +500,24 @@ template <typename Real, typename KReal>
 static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
     const int pr = sizeof(Real) == 4 ? 0 : 1;
     if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
-    for (int k = 0; k < 5; ++k) pass_impl<Real, KReal>(k, ls, tw, xf, xi, gate, st);
+    if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_off[6] > 0) {
+        // only the tiles of jobs that wait for their reliable update
+        TDDA_HIP(hipMemsetAsync(ls.ru_count, 0, 6 * sizeof(int), st));
+        RuRegions rg;
+        for (int k = 0; k < 7; ++k) rg.off[k] = ls.ru_off[k];
+        hipLaunchKernelGGL(k_ru_compact, dim3((ls.ru_off[6] + 255) / 256), dim3(256), 0, st, ls.jobs, ls.ru_src, rg,
+                           ls.ru_map, ls.ru_count, ls.ru_jobs);
+        LaunchSet lr = ls;
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) {
+                PassMap& pm = lr.pm[1][p][k];
+                pm.map = ls.ru_map + ls.ru_off[2 * p + k];
+                pm.limit = ls.ru_count + 2 * p + k;
+            }
+        for (int k = 0; k < 5; ++k) pass_impl<Real, KReal>(k, lr, tw, xf, xi, gate, st);
+    } else {
+        for (int k = 0; k < 5; ++k) pass_impl<Real, KReal>(k, ls, tw, xf, xi, gate, st);
+    }
     TDDA_HIP(hipGetLastError());
 }
 

@@ -21,11 +21,11 @@ __device__ inline void load_twiddles(C* twl, const C* __restrict__ twg, int L) {
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
-            int gate, const int* __restrict__ ru_jobs) {
+            int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     extern __shared__ __align__(16) unsigned char smem_xff[];
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -61,11 +61,11 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
 template <typename Real, typename KReal, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
-            int gate, const int* __restrict__ ru_jobs) {
+            int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     using KC = cx_t<KReal>;
     extern __shared__ __align__(16) unsigned char smem_xif[];
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -108,10 +108,10 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
 template <typename Real, int DIR>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
-             int gate, const int* __restrict__ ru_jobs) {
+             int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     extern __shared__ __align__(16) unsigned char smem_yf[];
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -149,10 +149,10 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
 template <typename Real>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
-             int gate, const int* __restrict__ ru_jobs) {
+             int gate, const int* __restrict__ limit) {
     using C = cx_t<Real>;
     extern __shared__ __align__(16) unsigned char smem_zf[];
-    if (ru_idle(gate, ru_jobs)) return;
+    if (beyond_limit(limit)) return;
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
@@ -189,27 +189,27 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
 }
 
 template <typename Real, typename KReal>
-void xfwd_fast(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int mode, int gate, const int* ru,
+void xfwd_fast(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int mode, int gate,
                hipStream_t st) {
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (mode) {
-        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_P: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_T: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XF_SOL: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XF_PLAIN: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_P: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_P>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_T: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_T>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XF_SOL: hipLaunchKernelGGL((k_xfwd_fast<Real, KReal, XF_SOL>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
         default: throw Error(ERR_INVALID, "bad X-forward mode");
     }
 }
 
 template <typename Real, typename KReal>
-void xinv_fast(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int mode, int gate, const int* ru,
+void xinv_fast(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int mode, int gate,
                hipStream_t st) {
     const dim3 grid(pm.n), blk(FFT_THREADS);
     switch (mode) {
-        case XI_PLAIN: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_V: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_S: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
-        case XI_RESID: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, ru); break;
+        case XI_PLAIN: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_PLAIN>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_V: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_V>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_S: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_S>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
+        case XI_RESID: hipLaunchKernelGGL((k_xinv_fast<Real, KReal, XI_RESID>), grid, blk, pm.lds, st, jobs, pm.map, tw, gate, pm.limit); break;
         default: throw Error(ERR_INVALID, "bad X-inverse mode");
     }
 }
@@ -219,40 +219,40 @@ void xinv_fast(const PassMap& pm, const GridJob* jobs, const cx_t<Real>* tw, int
 // Krylov precision follows the transform precision except for FP64 transforms
 // feeding c64 Krylov vectors (reliable updates in refined mode).
 void launch_fast_xfwd(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, const int* ru, hipStream_t st) {
+                      int mode, int gate, hipStream_t st) {
     if (pm.n == 0) return;
-    if (c64) xfwd_fast<float, float>(pm, jobs, tw.tw32, mode, gate, ru, st);
-    else if (krylov32) xfwd_fast<double, float>(pm, jobs, tw.tw64, mode, gate, ru, st);
-    else xfwd_fast<double, double>(pm, jobs, tw.tw64, mode, gate, ru, st);
+    if (c64) xfwd_fast<float, float>(pm, jobs, tw.tw32, mode, gate, st);
+    else if (krylov32) xfwd_fast<double, float>(pm, jobs, tw.tw64, mode, gate, st);
+    else xfwd_fast<double, double>(pm, jobs, tw.tw64, mode, gate, st);
 }
 
 void launch_fast_xinv(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, bool krylov32,
-                      int mode, int gate, const int* ru, hipStream_t st) {
+                      int mode, int gate, hipStream_t st) {
     if (pm.n == 0) return;
-    if (c64) xinv_fast<float, float>(pm, jobs, tw.tw32, mode, gate, ru, st);
-    else if (krylov32) xinv_fast<double, float>(pm, jobs, tw.tw64, mode, gate, ru, st);
-    else xinv_fast<double, double>(pm, jobs, tw.tw64, mode, gate, ru, st);
+    if (c64) xinv_fast<float, float>(pm, jobs, tw.tw32, mode, gate, st);
+    else if (krylov32) xinv_fast<double, float>(pm, jobs, tw.tw64, mode, gate, st);
+    else xinv_fast<double, double>(pm, jobs, tw.tw64, mode, gate, st);
 }
 
 void launch_fast_y(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int dir, int gate,
-                   const int* ru, hipStream_t st) {
+                   hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
     if (c64) {
-        if (dir < 0) hipLaunchKernelGGL((k_ypass_fast<float, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, ru);
-        else hipLaunchKernelGGL((k_ypass_fast<float, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, ru);
+        if (dir < 0) hipLaunchKernelGGL((k_ypass_fast<float, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, pm.limit);
+        else hipLaunchKernelGGL((k_ypass_fast<float, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, pm.limit);
     } else {
-        if (dir < 0) hipLaunchKernelGGL((k_ypass_fast<double, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, ru);
-        else hipLaunchKernelGGL((k_ypass_fast<double, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, ru);
+        if (dir < 0) hipLaunchKernelGGL((k_ypass_fast<double, -1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, pm.limit);
+        else hipLaunchKernelGGL((k_ypass_fast<double, 1>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, pm.limit);
     }
 }
 
 void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int gate,
-                   const int* ru, hipStream_t st) {
+                   hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
-    if (c64) hipLaunchKernelGGL((k_zpass_fast<float>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, ru);
-    else hipLaunchKernelGGL((k_zpass_fast<double>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, ru);
+    if (c64) hipLaunchKernelGGL((k_zpass_fast<float>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, pm.limit);
+    else hipLaunchKernelGGL((k_zpass_fast<double>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, pm.limit);
 }
 
 // tile geometry of the specialised path (host)

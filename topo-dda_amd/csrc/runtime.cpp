@@ -566,6 +566,19 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     ls.map_upd = base + off_u;
     ls.n_upd = (int)mu.size();
     ls.refined = refined;
+    // reliable-update compaction regions: the c128 maps, contiguous from
+    // pm[1][X][0] in build order
+    if (refined) {
+        const size_t n64 = off_u - off[1][0][0];
+        if (ru_maps_.size() < n64) ru_maps_.alloc(std::max<size_t>(n64, 1));
+        if (ru_count_.size() < 6) ru_count_.alloc(6);
+        ls.ru_map = ru_maps_.get();
+        ls.ru_count = ru_count_.get();
+        ls.ru_src = base + off[1][0][0];
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) ls.ru_off[2 * p + k] = (int)(off[1][p][k] - off[1][0][0]);
+        ls.ru_off[6] = (int)n64;
+    }
 }
 
 void Batch::upload_jobs(hipStream_t st) {

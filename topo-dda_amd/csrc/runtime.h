@@ -205,7 +205,8 @@ class Batch {
 
    private:
     DevBuf<GridJob> djobs_;
-    DevBuf<int2> maps_;
+    DevBuf<int2> maps_, ru_maps_;
+    DevBuf<int> ru_count_;
     std::vector<int2> hmaps_;
 };
 
