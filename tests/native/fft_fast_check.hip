@@ -154,6 +154,11 @@ int main() {
     // small DFTs
     check_len<4, 4>(); check_len<5, 4>(); check_len<6, 4>(); check_len<8, 4>(); check_len<8, 5>();
     check_len<8, 6>(); check_len<8, 8>(); check_len<10, 8>(); check_len<12, 8>(); check_len<16, 8>();
+    // Z-pass splits with L2 = 4
+    for (int n : {20, 19, 22, 40, 1}) { check_z<10, 4, float>(n); check_z<10, 4, double>(n); }
+    for (int n : {24, 23, 26, 48, 1}) { check_z<12, 4, float>(n); check_z<12, 4, double>(n); }
+    for (int n : {32, 31, 34, 64, 1}) { check_z<16, 4, float>(n); check_z<16, 4, double>(n); }
+    printf("Z splits with L2 = 4 done (worst so far %.3e)\n", worst);
     printf("worst rel err %.3e\n", worst);
     return worst > 1e-5 ? 1 : 0;
 }

@@ -380,4 +380,17 @@ TDDA_HD bool with_split(int L, F&& f) {
     }
 }
 
+// Splits of the Z pass: the multiply step holds 3 x L2 values per thread and
+// the tile holds 3 x L2 entries per thread in LDS, so L2 = 4 wherever L1 stays
+// within a 16-point register DFT (more resident waves per CU).
+template <typename F>
+TDDA_HD bool with_split_z(int L, F&& f) {
+    switch (L) {
+        case 40: f(IC<10>{}, IC<4>{}); return true;
+        case 48: f(IC<12>{}, IC<4>{}); return true;
+        case 64: f(IC<16>{}, IC<4>{}); return true;
+        default: return with_split(L, f);
+    }
+}
+
 }  // namespace tdda

@@ -156,7 +156,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    with_split(J.lz, [&](auto s1, auto s2) {
+    with_split_z(J.lz, [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_z<Real>(L1, L2);
         using Z = ZFast<L1, L2, NP, C>;
@@ -257,7 +257,7 @@ void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& t
 
 // tile geometry of the specialised path (host)
 bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
-    return with_split(L, [&](auto s1, auto s2) {
+    auto tiles = [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, LL = L1 * L2;
         const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
         if (pass == PASS_X) {
@@ -270,7 +270,8 @@ bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
             *per_tile = c64 ? fast_np_z<float>(L1, L2) : fast_np_z<double>(L1, L2);
             *lds_bytes = ((size_t)3 * *per_tile * LL + LL) * cs;
         }
-    });
+    };
+    return pass == PASS_Z ? with_split_z(L, tiles) : with_split(L, tiles);
 }
 
 }  // namespace tdda
