@@ -54,6 +54,7 @@ struct LaunchSet {
     int2* ru_map = nullptr;
     int* ru_count = nullptr;
     const int2* ru_src = nullptr;    // the c128 maps, contiguous, same regions
+    const int* ru_ranges = nullptr;  // [6][njobs + 1]: start of job j's tiles in region k
     int ru_off[7] = {0, 0, 0, 0, 0, 0, 0};             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
     const int2* map_upd = nullptr;
     int n_upd = 0;

@@ -317,23 +317,33 @@ __global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
 
 // ----------------------------------------------------------------------------
 // compaction of the c128 pass maps to the tiles of jobs waiting for a
-// reliable update (unordered within a region; counts zeroed by the caller)
+// reliable update (jobs in any order, each job's tiles contiguous; counts
+// zeroed by the caller)
 // ----------------------------------------------------------------------------
 struct RuRegions {
     int off[7];
 };
-__global__ void k_ru_compact(const GridJob* __restrict__ jobs, const int2* __restrict__ src, RuRegions rg,
-                             int2* __restrict__ dst, int* __restrict__ count, const int* __restrict__ ru_jobs) {
+__global__ void __launch_bounds__(FFT_THREADS)
+k_ru_compact(const GridJob* __restrict__ jobs, int njobs, const int2* __restrict__ src, RuRegions rg,
+             const int* __restrict__ ranges, int2* __restrict__ dst, int* __restrict__ count,
+             const int* __restrict__ ru_jobs) {
+    // one block per job; its tiles are contiguous in every region of src
     if (__hip_atomic_load(ru_jobs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rg.off[6]) return;
-    const int2 e = src[i];
-    const KScalars* ks = jobs[e.x].ks;
+    const int j = blockIdx.x;
+    const KScalars* ks = jobs[j].ks;
     if (ks->status != JOB_ACTIVE || ks->need_ru == 0) return;
-    int k = 0;
-    while (k < 5 && i >= rg.off[k + 1]) ++k;
-    const int slot = atomicAdd(&count[k], 1);
-    dst[rg.off[k] + slot] = e;
+    __shared__ int base[6];
+    if (threadIdx.x < 6) {
+        const int* r = ranges + threadIdx.x * (njobs + 1);
+        const int n = r[j + 1] - r[j];
+        base[threadIdx.x] = n > 0 ? atomicAdd(&count[threadIdx.x], n) : 0;
+    }
+    __syncthreads();
+    for (int k = 0; k < 6; ++k) {
+        const int* r = ranges + k * (njobs + 1);
+        const int s0 = r[j], n = r[j + 1] - s0;
+        for (int t = threadIdx.x; t < n; t += FFT_THREADS) dst[rg.off[k] + base[k] + t] = src[rg.off[k] + s0 + t];
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -500,13 +510,13 @@ template <typename Real, typename KReal>
 static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
     const int pr = sizeof(Real) == 4 ? 0 : 1;
     if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
-    if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_off[6] > 0) {
+    if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_ranges != nullptr && ls.ru_off[6] > 0) {
         // only the tiles of jobs that wait for their reliable update
         TDDA_HIP(hipMemsetAsync(ls.ru_count, 0, 6 * sizeof(int), st));
         RuRegions rg;
         for (int k = 0; k < 7; ++k) rg.off[k] = ls.ru_off[k];
-        hipLaunchKernelGGL(k_ru_compact, dim3((ls.ru_off[6] + 255) / 256), dim3(256), 0, st, ls.jobs, ls.ru_src, rg,
-                           ls.ru_map, ls.ru_count, ls.ru_jobs);
+        hipLaunchKernelGGL(k_ru_compact, dim3(ls.njobs), dim3(FFT_THREADS), 0, st, ls.jobs, ls.njobs, ls.ru_src, rg,
+                           ls.ru_ranges, ls.ru_map, ls.ru_count, ls.ru_jobs);
         LaunchSet lr = ls;
         for (int p = 0; p < 3; ++p)
             for (int k = 0; k < 2; ++k) {

@@ -516,6 +516,17 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         else groups[it->second].push_back((int)j);
     }
     for (int pr = 0; pr < 2; ++pr) {
+        if (pr == 1 && refined) {
+            // c128 maps of a refined batch serve only reliable updates, which
+            // run over per-job compacted copies: keep each job's tiles
+            // contiguous and in job order
+            for (size_t j = 0; j < hjobs.size(); ++j) {
+                const GridJob& J = hjobs[j];
+                const std::vector<int> order = z_tile_order(J, J.bz64, J.tz64);
+                for (int t : order) m[1][PASS_Z][J.fz].push_back(make_int2((int)j, t));
+            }
+            continue;
+        }
         for (const auto& g : groups) {
             const GridJob& J0 = hjobs[g[0]];
             const int T = pr == 0 ? J0.tz32 : J0.tz64;
@@ -578,6 +589,23 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         for (int p = 0; p < 3; ++p)
             for (int k = 0; k < 2; ++k) ls.ru_off[2 * p + k] = (int)(off[1][p][k] - off[1][0][0]);
         ls.ru_off[6] = (int)n64;
+        // per-region start of every job's (contiguous) tiles, [6][njobs + 1]
+        const int nj = (int)hjobs.size();
+        std::vector<int>& rr = hranges_;   // kept alive: the upload is asynchronous
+        rr.assign(6 * (nj + 1), 0);
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) {
+                const std::vector<int2>& v = m[1][p][k];
+                int* r = rr.data() + (2 * p + k) * (nj + 1);
+                size_t i = 0;
+                for (int j = 0; j <= nj; ++j) {
+                    while (i < v.size() && v[i].x < j) ++i;
+                    r[j] = (int)i;
+                }
+            }
+        if (ru_ranges_.size() < rr.size()) ru_ranges_.alloc(rr.size());
+        ru_ranges_.upload(rr.data(), rr.size(), st);
+        ls.ru_ranges = ru_ranges_.get();
     }
 }
 

@@ -206,7 +206,8 @@ class Batch {
    private:
     DevBuf<GridJob> djobs_;
     DevBuf<int2> maps_, ru_maps_;
-    DevBuf<int> ru_count_;
+    DevBuf<int> ru_count_, ru_ranges_;
+    std::vector<int> hranges_;
     std::vector<int2> hmaps_;
 };
 
