@@ -50,6 +50,7 @@ struct GridJob {
     int bx64, by64, bz64;      // same for c128 launches
     int tx32, ty32, tz32;      // tiles per pass, c64
     int tx64, ty64, tz64;      // tiles per pass, c128
+    int bxf32, bxf64, txf32, txf64;   // rows per tile / tiles of the X forward pass
     int tupd;                  // tiles of the elementwise update kernel
     int fx, fy, fz;            // 1: the pass uses the length-specialised kernels
     unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients

@@ -380,6 +380,19 @@ TDDA_HD bool with_split(int L, F&& f) {
     }
 }
 
+// Splits of the X and Y passes: a tile holds about L1 values per thread in
+// LDS, and every block of a launch reserves the largest tile of the launch,
+// so L1 = 8 for the long lengths (more resident waves per CU).
+template <typename F>
+TDDA_HD bool with_split_xy(int L, F&& f) {
+    switch (L) {
+        case 80: f(IC<8>{}, IC<10>{}); return true;
+        case 96: f(IC<8>{}, IC<12>{}); return true;
+        case 128: f(IC<8>{}, IC<16>{}); return true;
+        default: return with_split(L, f);
+    }
+}
+
 // Splits of the Z pass: the multiply step holds 3 x L2 values per thread and
 // the tile holds 3 x L2 entries per thread in LDS, so L2 = 4 wherever L1 stays
 // within a 16-point register DFT (more resident waves per CU).

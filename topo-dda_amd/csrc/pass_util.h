@@ -34,6 +34,7 @@ __device__ __forceinline__ void gstore(void* base, unsigned i, T x) {
 template <typename Real> struct Tile;
 template <> struct Tile<float> {
     __device__ static int bx(const GridJob& J) { return J.bx32; }
+    __device__ static int bxf(const GridJob& J) { return J.bxf32; }
     __device__ static int by(const GridJob& J) { return J.by32; }
     __device__ static int bz(const GridJob& J) { return J.bz32; }
     __device__ static int tx(const GridJob& J) { return J.tx32; }
@@ -42,6 +43,7 @@ template <> struct Tile<float> {
 };
 template <> struct Tile<double> {
     __device__ static int bx(const GridJob& J) { return J.bx64; }
+    __device__ static int bxf(const GridJob& J) { return J.bxf64; }
     __device__ static int by(const GridJob& J) { return J.by64; }
     __device__ static int bz(const GridJob& J) { return J.bz64; }
     __device__ static int tx(const GridJob& J) { return J.tx64; }

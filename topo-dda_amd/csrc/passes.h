@@ -35,7 +35,11 @@ struct PassMap {
     size_t lds = 0;
     const int* limit = nullptr;   // device count of valid map entries (blocks beyond it return)
 };
-enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
+// PASS_X: X inverse tiles (its dot partials are per tile); PASS_XF: X forward
+// tiles, sized apart from the inverse.
+enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2, PASS_XF = 3 };
+constexpr int NPASS = 4;
+constexpr int RU_REGIONS = 2 * NPASS;
 
 // A launch over a batch: device array of jobs and, per transform precision
 // and pass, the blocks of the generic kernels and of the length-specialised
@@ -44,18 +48,18 @@ enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2 };
 struct LaunchSet {
     const GridJob* jobs = nullptr;   // device
     int njobs = 0;
-    PassMap pm[2][3][2];
+    PassMap pm[2][NPASS][2];
     // device count of jobs (of any batch) that wait for a reliable update
     int* ru_jobs = nullptr;
     // reliable-update launches run over compacted copies of the c128 maps
-    // (only tiles of waiting jobs): ru_map holds the six regions in the order
-    // of pm[1] (X gen, X spec, Y gen, Y spec, Z gen, Z spec) at ru_off[k],
+    // (only tiles of waiting jobs): ru_map holds the RU_REGIONS regions in
+    // the order of pm[1] (pass-major, generic then specialised) at ru_off[k],
     // ru_count[k] their filled lengths
     int2* ru_map = nullptr;
     int* ru_count = nullptr;
     const int2* ru_src = nullptr;    // the c128 maps, contiguous, same regions
-    const int* ru_ranges = nullptr;  // [6][njobs + 1]: start of job j's tiles in region k
-    int ru_off[7] = {0, 0, 0, 0, 0, 0, 0};             // [0: c64, 1: c128][X, Y, Z][0: generic, 1: specialised]
+    const int* ru_ranges = nullptr;  // [RU_REGIONS][njobs + 1]: start of job j's tiles in region k
+    int ru_off[RU_REGIONS + 1] = {};
     const int2* map_upd = nullptr;
     int n_upd = 0;
     bool refined = false;            // Krylov vectors are c64

@@ -34,7 +34,7 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    const int bx = Tile<Real>::bx(J);
+    const int bx = Tile<Real>::bxf(J);
     const int nx = J.nx, ny = J.ny, lx = J.lx;
     const int nrows = ny * J.nz;
     const int q0 = jt.y * bx;
@@ -321,7 +321,7 @@ __global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
 // zeroed by the caller)
 // ----------------------------------------------------------------------------
 struct RuRegions {
-    int off[7];
+    int off[RU_REGIONS + 1];
 };
 __global__ void __launch_bounds__(FFT_THREADS)
 k_ru_compact(const GridJob* __restrict__ jobs, int njobs, const int2* __restrict__ src, RuRegions rg,
@@ -332,14 +332,14 @@ k_ru_compact(const GridJob* __restrict__ jobs, int njobs, const int2* __restrict
     const int j = blockIdx.x;
     const KScalars* ks = jobs[j].ks;
     if (ks->status != JOB_ACTIVE || ks->need_ru == 0) return;
-    __shared__ int base[6];
-    if (threadIdx.x < 6) {
+    __shared__ int base[RU_REGIONS];
+    if (threadIdx.x < RU_REGIONS) {
         const int* r = ranges + threadIdx.x * (njobs + 1);
         const int n = r[j + 1] - r[j];
         base[threadIdx.x] = n > 0 ? atomicAdd(&count[threadIdx.x], n) : 0;
     }
     __syncthreads();
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < RU_REGIONS; ++k) {
         const int* r = ranges + k * (njobs + 1);
         const int s0 = r[j], n = r[j + 1] - s0;
         for (int t = threadIdx.x; t < n; t += FFT_THREADS) dst[rg.off[k] + base[k] + t] = src[rg.off[k] + s0 + t];
@@ -478,8 +478,8 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
     const cx_t<Real>* t = c64 ? reinterpret_cast<const cx_t<Real>*>(tw.tw32) : reinterpret_cast<const cx_t<Real>*>(tw.tw64);
     switch (k) {
         case 0:
-            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xf, gate, st);
-            launch_fast_xfwd(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xf, gate, st);
+            xfwd_generic<Real, KReal>(ls.pm[pr][PASS_XF][0], ls.jobs, t, xf, gate, st);
+            launch_fast_xfwd(ls.pm[pr][PASS_XF][1], ls.jobs, tw, c64, k32, xf, gate, st);
             break;
         case 1:
         case 3:
@@ -510,15 +510,15 @@ template <typename Real, typename KReal>
 static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
     const int pr = sizeof(Real) == 4 ? 0 : 1;
     if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
-    if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_ranges != nullptr && ls.ru_off[6] > 0) {
+    if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_ranges != nullptr && ls.ru_off[RU_REGIONS] > 0) {
         // only the tiles of jobs that wait for their reliable update
-        TDDA_HIP(hipMemsetAsync(ls.ru_count, 0, 6 * sizeof(int), st));
+        TDDA_HIP(hipMemsetAsync(ls.ru_count, 0, RU_REGIONS * sizeof(int), st));
         RuRegions rg;
-        for (int k = 0; k < 7; ++k) rg.off[k] = ls.ru_off[k];
+        for (int k = 0; k <= RU_REGIONS; ++k) rg.off[k] = ls.ru_off[k];
         hipLaunchKernelGGL(k_ru_compact, dim3(ls.njobs), dim3(FFT_THREADS), 0, st, ls.jobs, ls.njobs, ls.ru_src, rg,
                            ls.ru_ranges, ls.ru_map, ls.ru_count, ls.ru_jobs);
         LaunchSet lr = ls;
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < NPASS; ++p)
             for (int k = 0; k < 2; ++k) {
                 PassMap& pm = lr.pm[1][p][k];
                 pm.map = ls.ru_map + ls.ru_off[2 * p + k];
@@ -615,9 +615,13 @@ void plan_tiles(GridJob& J) {
     J.fx = fast_tiles(J.lx, PASS_X, true, &J.bx32, &unused) ? 1 : 0;
     if (J.fx) {
         fast_tiles(J.lx, PASS_X, false, &J.bx64, &unused);
+        fast_tiles(J.lx, PASS_XF, true, &J.bxf32, &unused);
+        fast_tiles(J.lx, PASS_XF, false, &J.bxf64, &unused);
     } else {
         J.bx32 = std::max(1, Caps<float>::X / (3 * J.lx));
         J.bx64 = std::max(1, Caps<double>::X / (3 * J.lx));
+        J.bxf32 = J.bx32;
+        J.bxf64 = J.bx64;
     }
     J.fy = fast_tiles(J.ly, PASS_Y, true, &J.by32, &unused) ? 1 : 0;
     if (J.fy) {
@@ -635,6 +639,8 @@ void plan_tiles(GridJob& J) {
     }
     const long nrows = (long)J.ny * J.nz;
     J.tx32 = cdiv(nrows, J.bx32);
+    J.txf32 = cdiv(nrows, J.bxf32);
+    J.txf64 = cdiv(nrows, J.bxf64);
     J.ty32 = cdiv(3L * J.nz * J.lx, J.by32);
     J.tz32 = cdiv((long)J.ly * J.lx, J.bz32);
     J.tx64 = cdiv(nrows, J.bx64);
@@ -644,8 +650,8 @@ void plan_tiles(GridJob& J) {
 }
 
 size_t lds_bytes(const GridJob& J, int pass, bool c64) {
-    const int L = pass == PASS_X ? J.lx : pass == PASS_Y ? J.ly : J.lz;
-    const bool fast = pass == PASS_X ? J.fx : pass == PASS_Y ? J.fy : J.fz;
+    const int L = (pass == PASS_X || pass == PASS_XF) ? J.lx : pass == PASS_Y ? J.ly : J.lz;
+    const bool fast = (pass == PASS_X || pass == PASS_XF) ? J.fx : pass == PASS_Y ? J.fy : J.fz;
     const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
     if (fast) {
         int per = 0;
@@ -654,6 +660,7 @@ size_t lds_bytes(const GridJob& J, int pass, bool c64) {
         return bytes;
     }
     if (pass == PASS_X) return (size_t)J.lx * ((3 * (c64 ? J.bx32 : J.bx64)) | 1) * cs;
+    if (pass == PASS_XF) return (size_t)J.lx * ((3 * (c64 ? J.bxf32 : J.bxf64)) | 1) * cs;
     if (pass == PASS_Y) return (size_t)J.ly * ((c64 ? J.by32 : J.by64) | 1) * cs;
     return (size_t)3 * J.lz * ((c64 ? J.bz32 : J.bz64) | 1) * cs;
 }

@@ -29,7 +29,7 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    with_split(J.lx, [&](auto s1, auto s2) {
+    with_split_xy(J.lx, [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         using X = XFast<L1, L2, fast_nr_x<Real>(L1, L2), C>;
         C* lds = reinterpret_cast<C*>(smem_xff);
@@ -260,7 +260,7 @@ bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
     auto tiles = [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, LL = L1 * L2;
         const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
-        if (pass == PASS_X) {
+        if (pass == PASS_X || pass == PASS_XF) {
             *per_tile = c64 ? fast_nr_x<float>(L1, L2) : fast_nr_x<double>(L1, L2);
             *lds_bytes = ((size_t)3 * *per_tile * fast_ps_x(L1, L2) + LL) * cs;
         } else if (pass == PASS_Y) {
@@ -271,7 +271,9 @@ bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
             *lds_bytes = ((size_t)3 * *per_tile * LL + LL) * cs;
         }
     };
-    return pass == PASS_Z ? with_split_z(L, tiles) : with_split(L, tiles);
+    if (pass == PASS_Z) return with_split_z(L, tiles);
+    if (pass == PASS_XF) return with_split_xy(L, tiles);
+    return with_split(L, tiles);
 }
 
 }  // namespace tdda

@@ -480,9 +480,9 @@ void xcd_permute(std::vector<int2>& m) {
 void Batch::build(const std::vector<System*>& systems, const std::vector<KScalars*>& ks, bool refined,
                   hipStream_t st, bool interleave_shared) {
     hjobs.clear();
-    std::vector<int2> m[2][3][2];
+    std::vector<int2> m[2][NPASS][2];
     std::vector<int2> mu;
-    size_t lds[2][3][2] = {};
+    size_t lds[2][NPASS][2] = {};
     for (size_t j = 0; j < systems.size(); ++j) {
         System* s = systems[j];
         TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
@@ -494,6 +494,10 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             const bool c64 = pr == 0;
             const int tx = c64 ? J.tx32 : J.tx64, ty = c64 ? J.ty32 : J.ty64;
             for (int t = 0; t < tx; ++t) m[pr][PASS_X][J.fx].push_back(make_int2(jj, t));
+            const int txf = c64 ? J.txf32 : J.txf64;
+            for (int t = 0; t < txf; ++t) m[pr][PASS_XF][J.fx].push_back(make_int2(jj, t));
+            size_t& lxf = lds[pr][PASS_XF][J.fx];
+            lxf = std::max(lxf, lds_bytes(J, PASS_XF, c64));
             for (int t = 0; t < ty; ++t) m[pr][PASS_Y][J.fy].push_back(make_int2(jj, t));
             size_t& lx = lds[pr][PASS_X][J.fx];
             size_t& ly = lds[pr][PASS_Y][J.fy];
@@ -548,9 +552,9 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     std::vector<int2>& all = hmaps_;   // kept alive: the upload is asynchronous
     all.clear();
     all.reserve(total);
-    size_t off[2][3][2];
+    size_t off[2][NPASS][2];
     for (int pr = 0; pr < 2; ++pr)
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < NPASS; ++p)
             for (int k = 0; k < 2; ++k) {
                 off[pr][p][k] = all.size();
                 all.insert(all.end(), m[pr][p][k].begin(), m[pr][p][k].end());
@@ -567,7 +571,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     ls.njobs = (int)hjobs.size();
     ls.ru_jobs = ru_job_counter();
     for (int pr = 0; pr < 2; ++pr)
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < NPASS; ++p)
             for (int k = 0; k < 2; ++k) {
                 PassMap& pm = ls.pm[pr][p][k];
                 pm.map = base + off[pr][p][k];
@@ -582,18 +586,18 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     if (refined) {
         const size_t n64 = off_u - off[1][0][0];
         if (ru_maps_.size() < n64) ru_maps_.alloc(std::max<size_t>(n64, 1));
-        if (ru_count_.size() < 6) ru_count_.alloc(6);
+        if (ru_count_.size() < (size_t)RU_REGIONS) ru_count_.alloc(RU_REGIONS);
         ls.ru_map = ru_maps_.get();
         ls.ru_count = ru_count_.get();
         ls.ru_src = base + off[1][0][0];
-        for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < NPASS; ++p)
             for (int k = 0; k < 2; ++k) ls.ru_off[2 * p + k] = (int)(off[1][p][k] - off[1][0][0]);
-        ls.ru_off[6] = (int)n64;
-        // per-region start of every job's (contiguous) tiles, [6][njobs + 1]
+        ls.ru_off[RU_REGIONS] = (int)n64;
+        // per-region start of every job's (contiguous) tiles, [RU_REGIONS][njobs + 1]
         const int nj = (int)hjobs.size();
         std::vector<int>& rr = hranges_;   // kept alive: the upload is asynchronous
-        rr.assign(6 * (nj + 1), 0);
-        for (int p = 0; p < 3; ++p)
+        rr.assign(RU_REGIONS * (nj + 1), 0);
+        for (int p = 0; p < NPASS; ++p)
             for (int k = 0; k < 2; ++k) {
                 const std::vector<int2>& v = m[1][p][k];
                 int* r = rr.data() + (2 * p + k) * (nj + 1);
