@@ -73,7 +73,7 @@ template <int L1, int L2, typename Real> void check_z(int n) {
     std::vector<C> lds(Z::LDS);
     auto idx = [&](int c, int b, int pos) { return (c * NP + b) * L + pos; };
     for (int t = 0; t < Z::TA; ++t) Z::fwd_a(t, lds.data(), tw.data(), nb, n, [&](int c, int b, int pos) { return g[idx(c, b, pos)]; });
-    for (int t = 0; t < Z::TB; ++t) Z::mid_b(t, lds.data(), tw.data(), nb, [&](int b, int kz, C& v0, C& v1, C& v2) {
+    for (int t = 0; t < Z::TB; ++t) Z::mid_b(t, lds.data(), tw.data(), nb, [&](int b, int kz, auto, C& v0, C& v1, C& v2) {
         auto gg = [&](int k) { cld z = G[(k * NP + b) * L + kz]; return mk<C>((double)z.real(), (double)z.imag()); };
         C a0 = v0, a1 = v1, a2 = v2;
         v0 = cadd(cadd(cmul(gg(0), a0), cmul(gg(1), a1)), cmul(gg(2), a2));

@@ -56,7 +56,8 @@ struct GridJob {
     unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients
     const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
     // operator data
-    const float2* green32;     // [hz][hy][hx][6] (octant spectrum, point-major)
+    const float2* green32;     // octant spectrum, [hz][hy][3][hx] of component pairs
+                               // (FP64: [hz][hy][6][hx])
     const double2* green64;
     const float2* al32;        // [ndip] 1/alpha
     const double2* al64;

@@ -172,8 +172,9 @@ struct ZFast {
         });
     }
 
-    // step B forward + multiply (mul(b, kz, v0, v1, v2)) + step B inverse,
-    // in place: the task reads and writes the same LDS entries.
+    // step B forward + multiply (mul(b, kz, IC<k2>, v0, v1, v2), kz = k1 +
+    // L1 k2) + step B inverse, in place: the task reads and writes the same
+    // LDS entries.
     template <typename Mul>
     TDDA_HD static void mid_b(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Mul mul) {
         const int b = tau % NP;
@@ -190,7 +191,7 @@ struct ZFast {
         });
         static_for<L2>([&](auto i) {
             constexpr int k2 = decltype(i)::value;
-            mul(b, k1 + L1 * k2, v[0][k2], v[1][k2], v[2][k2]);
+            mul(b, k1 + L1 * k2, i, v[0][k2], v[1][k2], v[2][k2]);
         });
         static_for<3>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
@@ -404,6 +405,19 @@ TDDA_HD bool with_split_z(int L, F&& f) {
         case 64: f(IC<16>{}, IC<4>{}); return true;
         default: return with_split(L, f);
     }
+}
+
+// Z lengths fall in two kernel classes: 0 for the splits with L2 = 4, 1 for
+// the others (their multiply step holds 3 x L2 >= 15 values per thread).
+// Each class is its own kernel, so the register budget of the short splits
+// is not set by the long ones.
+template <typename S2>
+constexpr int z_class_of() { return S2::value == 4 ? 0 : 1; }
+// -1: no specialised split
+inline int z_class(int L) {
+    int c = -1;
+    with_split_z(L, [&](auto, auto s2) { c = z_class_of<decltype(s2)>(); });
+    return c;
 }
 
 }  // namespace tdda

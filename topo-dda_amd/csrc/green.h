@@ -21,8 +21,10 @@ void launch_green_octant(const GreenParams& P, const double* si, const double* c
 // one parity-reduced DFT axis: in [6][outer][no][inner] -> out [6][outer][h][inner]
 void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
                       const double2* tw, hipStream_t st);
-// [6][npts] component-major spectrum -> point-major [npts][6] in FP64 and c64
-// (the layout the Z passes read: one contiguous 6-component record per point)
-void launch_green_interleave(const double2* in, long npts, double2* out64, float2* out32, hipStream_t st);
+// [6][rows][hx] component-major spectrum (rows = hz * hy) -> the layouts the
+// Z passes read, where a run of consecutive kx is one contiguous range per
+// load: FP64 [rows][6][hx] (double2), c64 [rows][3][hx] of float4 holding
+// the component pairs (00,01), (02,11), (12,22).
+void launch_green_interleave(const double2* in, long rows, int hx, double2* out64, float2* out32, hipStream_t st);
 
 }  // namespace tdda

@@ -111,15 +111,17 @@ __global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ i
     out[(((long)c * outer + a) * h + k) * inner + i] = acc;
 }
 
-__global__ void k_green_interleave(const double2* __restrict__ in, long npts, double2* __restrict__ out64,
+__global__ void k_green_interleave(const double2* __restrict__ in, long rows, int hx, double2* __restrict__ out64,
                                    float2* __restrict__ out32) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 6 * npts) return;
-    const long p = i / 6;
-    const int c = (int)(i - 6 * p);
-    const double2 v = in[c * npts + p];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;   // over [rows][6][hx]
+    if (i >= 6 * rows * hx) return;
+    const int x = (int)(i % hx);
+    const long rc = i / hx;
+    const long row = rc / 6;
+    const int c = (int)(rc - 6 * row);
+    const double2 v = in[(c * rows + row) * hx + x];
     out64[i] = v;
-    out32[i] = make_float2((float)v.x, (float)v.y);
+    out32[((row * 3 + (c >> 1)) * hx + x) * 2 + (c & 1)] = make_float2((float)v.x, (float)v.y);
 }
 
 void launch_green_octant(const GreenParams& P, const double* si, const double* ci, double2* out, hipStream_t st) {
@@ -139,9 +141,9 @@ void launch_green_dft(const double2* in, double2* out, int outer, int no, int h,
     TDDA_HIP(hipGetLastError());
 }
 
-void launch_green_interleave(const double2* in, long npts, double2* out64, float2* out32, hipStream_t st) {
-    const long n = 6 * npts;
-    hipLaunchKernelGGL(k_green_interleave, dim3((n + 255) / 256), dim3(256), 0, st, in, npts, out64, out32);
+void launch_green_interleave(const double2* in, long rows, int hx, double2* out64, float2* out32, hipStream_t st) {
+    const long n = 6 * rows * hx;
+    hipLaunchKernelGGL(k_green_interleave, dim3((n + 255) / 256), dim3(256), 0, st, in, rows, hx, out64, out32);
     TDDA_HIP(hipGetLastError());
 }
 

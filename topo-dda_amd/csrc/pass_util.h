@@ -367,35 +367,50 @@ __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int
 // {y,z}, {} respectively, so only k <= L/2 per axis is stored and
 // Ghat(L-k) = +-Ghat(k).
 // ----------------------------------------------------------------------------
-// The six components of one spectral point are one contiguous record.
-__device__ inline void green_load6(const float2* G, unsigned off, float2 (&g)[6]) {
-    const float4 a = gload<float4>(G, 3u * off), b = gload<float4>(G, 3u * off + 1u),
-                 c = gload<float4>(G, 3u * off + 2u);
+// Spectrum rows (kz, ky) hold each component (FP64) or component pair (c64,
+// pairs (00,01), (02,11), (12,22) in one float4) as a run over kx, so the
+// lanes of a wave, which take consecutive kx, load contiguous bytes.
+__device__ inline void green_load6(const float2* G, unsigned row, unsigned hx, unsigned kx, float2 (&g)[6]) {
+    const unsigned o = 3u * row * hx + kx;
+    const float4 a = gload<float4>(G, o), b = gload<float4>(G, o + hx), c = gload<float4>(G, o + 2u * hx);
     g[0] = make_float2(a.x, a.y); g[1] = make_float2(a.z, a.w);
     g[2] = make_float2(b.x, b.y); g[3] = make_float2(b.z, b.w);
     g[4] = make_float2(c.x, c.y); g[5] = make_float2(c.z, c.w);
 }
-__device__ inline void green_load6(const double2* G, unsigned off, double2 (&g)[6]) {
+__device__ inline void green_load6(const double2* G, unsigned row, unsigned hx, unsigned kx, double2 (&g)[6]) {
+    const unsigned o = 6u * row * hx + kx;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) g[k] = gload<double2>(G, 6u * off + k);
+    for (int k = 0; k < 6; ++k) g[k] = gload<double2>(G, o + (unsigned)k * hx);
 }
 
+// The record of spectral point (kx, ky, kz) from the octant, with the
+// parity signs of the mirrored axes applied.
 template <typename C>
-__device__ inline void green_apply(const C* G, const GridJob& J, int kx, int ky, int kz, C& v0, C& v1, C& v2) {
+__device__ inline void green_fetch(const C* G, const GridJob& J, int kx, int ky, int kz, C (&g)[6]) {
     int sx = 1, sy = 1, sz = 1;
     if (2 * kx > J.lx) { kx = J.lx - kx; sx = -1; }
     if (2 * ky > J.ly) { ky = J.ly - ky; sy = -1; }
     if (2 * kz > J.lz) { kz = J.lz - kz; sz = -1; }
-    const unsigned off = ((unsigned)kz * J.hy + ky) * J.hx + kx;
-    C g[6];
-    green_load6(G, off, g);
+    green_load6(G, (unsigned)kz * J.hy + ky, J.hx, kx, g);
     if (sx * sy < 0) g[1] = cscale(g[1], -1);
     if (sx * sz < 0) g[2] = cscale(g[2], -1);
     if (sy * sz < 0) g[4] = cscale(g[4], -1);
+}
+
+// (v0, v1, v2) <- G (v0, v1, v2), G symmetric
+template <typename C>
+__device__ inline void green_mul(const C (&g)[6], C& v0, C& v1, C& v2) {
     const C a0 = v0, a1 = v1, a2 = v2;
     v0 = cadd(cadd(cmul(g[0], a0), cmul(g[1], a1)), cmul(g[2], a2));
     v1 = cadd(cadd(cmul(g[1], a0), cmul(g[3], a1)), cmul(g[4], a2));
     v2 = cadd(cadd(cmul(g[2], a0), cmul(g[4], a1)), cmul(g[5], a2));
+}
+
+template <typename C>
+__device__ inline void green_apply(const C* G, const GridJob& J, int kx, int ky, int kz, C& v0, C& v1, C& v2) {
+    C g[6];
+    green_fetch(G, J, kx, ky, kz, g);
+    green_mul(g, v0, v1, v2);
 }
 
 }  // namespace tdda

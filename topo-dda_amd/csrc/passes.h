@@ -34,6 +34,10 @@ struct PassMap {
     int n = 0;
     size_t lds = 0;
     const int* limit = nullptr;   // device count of valid map entries (blocks beyond it return)
+    // specialised Z pass: bit c is set when a job of length class c is in the
+    // map (each class is its own kernel), lds_cls[c] the LDS of that launch
+    int cls = 0;
+    size_t lds_cls[2] = {0, 0};
 };
 // PASS_X: X inverse tiles (its dot partials are per tile); PASS_XF: X forward
 // tiles, sized apart from the inverse.

@@ -146,7 +146,7 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
 // ----------------------------------------------------------------------------
 // Z pass: forward z FFT, Green's multiply, inverse z FFT (z < Nz in and out)
 // ----------------------------------------------------------------------------
-template <typename Real>
+template <typename Real, int CLS>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
              int gate, const int* __restrict__ limit) {
@@ -157,6 +157,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
     with_split_z(J.lz, [&](auto s1, auto s2) {
+        if constexpr (z_class_of<decltype(s2)>() != CLS) return;   // the other class's launch
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_z<Real>(L1, L2);
         using Z = ZFast<L1, L2, NP, C>;
@@ -177,7 +178,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
             Z::fwd_a(t, lds, twl, nb, nz, [&](int c, int b, int pos) -> C { return gload<C>(g, c * S + pos * plane + b); });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TB; t += FFT_THREADS)
-            Z::mid_b(t, lds, twl, nb, [&](int b, int kz, C& v0, C& v1, C& v2) {
+            Z::mid_b(t, lds, twl, nb, [&](int b, int kz, auto, C& v0, C& v1, C& v2) {
                 const unsigned p = pl0 + b;
                 const unsigned ky = fdiv(p, mlx);
                 green_apply(G, J, (int)(p - ky * lx), (int)ky, kz, v0, v1, v2);
@@ -251,8 +252,15 @@ void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& t
                    hipStream_t st) {
     if (pm.n == 0) return;
     const dim3 grid(pm.n), blk(FFT_THREADS);
-    if (c64) hipLaunchKernelGGL((k_zpass_fast<float>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw32, gate, pm.limit);
-    else hipLaunchKernelGGL((k_zpass_fast<double>), grid, blk, pm.lds, st, jobs, pm.map, tw.tw64, gate, pm.limit);
+    // one launch per length class present; blocks of the other class return
+    if (pm.cls & 1) {
+        if (c64) hipLaunchKernelGGL((k_zpass_fast<float, 0>), grid, blk, pm.lds_cls[0], st, jobs, pm.map, tw.tw32, gate, pm.limit);
+        else hipLaunchKernelGGL((k_zpass_fast<double, 0>), grid, blk, pm.lds_cls[0], st, jobs, pm.map, tw.tw64, gate, pm.limit);
+    }
+    if (pm.cls & 2) {
+        if (c64) hipLaunchKernelGGL((k_zpass_fast<float, 1>), grid, blk, pm.lds_cls[1], st, jobs, pm.map, tw.tw32, gate, pm.limit);
+        else hipLaunchKernelGGL((k_zpass_fast<double, 1>), grid, blk, pm.lds_cls[1], st, jobs, pm.map, tw.tw64, gate, pm.limit);
+    }
 }
 
 // tile geometry of the specialised path (host)

@@ -1,5 +1,6 @@
 // Host runtime implementation: memory, plans, operators, systems, batches.
 #include "runtime.h"
+#include "fft_fast.h"
 
 #include <algorithm>
 #include <atomic>
@@ -248,7 +249,7 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     launch_green_dft(b.get(), c.get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
     DevBuf<double2> e(6L * hz * hy * hx);
     launch_green_dft(c.get(), e.get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
-    launch_green_interleave(e.get(), (long)hz * hy * hx, green64.get(), green32.get(), st);
+    launch_green_interleave(e.get(), (long)hz * hy, hx, green64.get(), green32.get(), st);
     TDDA_HIP(hipStreamSynchronize(st));
     setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -483,6 +484,8 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     std::vector<int2> m[2][NPASS][2];
     std::vector<int2> mu;
     size_t lds[2][NPASS][2] = {};
+    int zcls[2] = {0, 0};                  // specialised Z length classes present
+    size_t zlds[2][2] = {};
     for (size_t j = 0; j < systems.size(); ++j) {
         System* s = systems[j];
         TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
@@ -505,6 +508,11 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             lx = std::max(lx, lds_bytes(J, PASS_X, c64));
             ly = std::max(ly, lds_bytes(J, PASS_Y, c64));
             lz = std::max(lz, lds_bytes(J, PASS_Z, c64));
+            if (J.fz) {
+                const int zc = z_class(J.lz);
+                zcls[pr] |= 1 << zc;
+                zlds[pr][zc] = std::max(zlds[pr][zc], lds_bytes(J, PASS_Z, c64));
+            }
         }
         for (int t = 0; t < J.tupd; ++t) mu.push_back(make_int2(jj, t));
     }
@@ -577,6 +585,11 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
                 pm.map = base + off[pr][p][k];
                 pm.n = (int)m[pr][p][k].size();
                 pm.lds = lds[pr][p][k];
+                if (p == PASS_Z && k == 1) {
+                    pm.cls = zcls[pr];
+                    pm.lds_cls[0] = zlds[pr][0];
+                    pm.lds_cls[1] = zlds[pr][1];
+                }
             }
     ls.map_upd = base + off_u;
     ls.n_upd = (int)mu.size();
@@ -653,7 +666,6 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
     launch_init(b.ls, opt.warm, st);
     const bool c64 = opt.refined;
     std::vector<KScalars> snap(nj);
-    int done_iters = 0;
     int chunk = opt.chunk > 0 ? opt.chunk : 4;
     while (true) {
         for (int k = 0; k < chunk; ++k) {
@@ -662,7 +674,6 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
             if (opt.refined) launch_matvec(b.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, st);
         }
         launch_finalize(b.ls, st);
-        done_iters += chunk;
         ks.download(snap.data(), nj, st);
         TDDA_HIP(hipStreamSynchronize(st));
         bool all_done = true;

@@ -118,7 +118,7 @@ class Operator {
     OperatorKey key;
     int nx, ny, nz, lx, ly, lz, hx, hy, hz;
     FftPlan px, py, pz;
-    DevBuf<double2> green64;   // [hz][hy][hx][6]
+    DevBuf<double2> green64;   // [hz][hy][6][hx] (c64: [hz][hy][3][hx] of component pairs)
     DevBuf<float2> green32;
     double setup_ms = 0.0;
     size_t bytes() const { return green64.bytes() + green32.bytes(); }
