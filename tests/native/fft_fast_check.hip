@@ -57,28 +57,32 @@ template <int L1, int L2, typename Real> void check_y(int n_in, int n_out, int d
     rep<C>("Y", L, e / nrm, sizeof(Real) == 4 ? 3e-6 : 1e-14);
 }
 
-template <int L1, int L2, typename Real> void check_z(int n) {
+template <int L1, int L2, typename Real, bool PAIR> void check_z_impl(int n) {
     using C = cx_t<Real>;
     constexpr int L = L1 * L2;
     constexpr int NP = fast_np_z<Real>(L1, L2);
-    using Z = ZFast<L1, L2, NP, C>;
+    using Z = ZFast<L1, L2, NP, C, PAIR>;
     auto tw = table<C>(L);
     int nb = NP - 3 > 0 ? NP - 3 : NP;
     std::vector<cld> X(3 * NP * L);
     for (auto& v : X) v = cld(urand(), urand());
+    // test spectrum with the z parity of the Green's tensor: components 02
+    // and 12 are odd in z, the others even
     std::vector<cld> G(6 * NP * L);
-    for (auto& v : G) v = cld(urand(), urand());
+    for (int q = 0; q < 6; ++q)
+        for (int b = 0; b < NP; ++b)
+            for (int k = 0; k < L; ++k) {
+                const int m = (L - k) % L;
+                G[(q * NP + b) * L + k] = k <= m ? cld(urand(), urand())
+                                                 : (q == 2 || q == 4 ? -1.0L : 1.0L) * G[(q * NP + b) * L + m];
+            }
     std::vector<C> g(3 * NP * L), out(3 * NP * L, mk<C>(7, 7));
     for (size_t i = 0; i < g.size(); ++i) g[i] = mk<C>((double)X[i].real(), (double)X[i].imag());
     std::vector<C> lds(Z::LDS);
     auto idx = [&](int c, int b, int pos) { return (c * NP + b) * L + pos; };
     for (int t = 0; t < Z::TA; ++t) Z::fwd_a(t, lds.data(), tw.data(), nb, n, [&](int c, int b, int pos) { return g[idx(c, b, pos)]; });
-    for (int t = 0; t < Z::TB; ++t) Z::mid_b(t, lds.data(), tw.data(), nb, [&](int b, int kz, auto, C& v0, C& v1, C& v2) {
-        auto gg = [&](int k) { cld z = G[(k * NP + b) * L + kz]; return mk<C>((double)z.real(), (double)z.imag()); };
-        C a0 = v0, a1 = v1, a2 = v2;
-        v0 = cadd(cadd(cmul(gg(0), a0), cmul(gg(1), a1)), cmul(gg(2), a2));
-        v1 = cadd(cadd(cmul(gg(1), a0), cmul(gg(3), a1)), cmul(gg(4), a2));
-        v2 = cadd(cadd(cmul(gg(2), a0), cmul(gg(4), a1)), cmul(gg(5), a2));
+    for (int t = 0; t < Z::TB; ++t) Z::mid(t, lds.data(), tw.data(), nb, [&](int b, int kz, C (&gg)[6]) {
+        for (int k = 0; k < 6; ++k) { cld z = G[(k * NP + b) * L + kz]; gg[k] = mk<C>((double)z.real(), (double)z.imag()); }
     });
     for (int t = 0; t < Z::TA; ++t) Z::inv_a(t, lds.data(), nb, n, [&](int c, int b, int pos, C v) { out[idx(c, b, pos)] = v; });
     double e = 0, nrm = 0;
@@ -99,7 +103,12 @@ template <int L1, int L2, typename Real> void check_z(int n) {
         }
     }
     for (int b = nb; b < NP; ++b) for (int c = 0; c < 3; ++c) if (out[idx(c, b, 0)].x != 7) { printf("Z wrote partial\n"); worst = 1; }
-    rep<C>("Z", L, e / nrm, sizeof(Real) == 4 ? 5e-6 : 1e-14);
+    rep<C>(PAIR ? "Z(pairs)" : "Z", L, e / nrm, sizeof(Real) == 4 ? 5e-6 : 1e-14);
+}
+// both multiply-step task layouts: z-mirror pairs and one task per k1
+template <int L1, int L2, typename Real> void check_z(int n) {
+    check_z_impl<L1, L2, Real, true>(n);
+    check_z_impl<L1, L2, Real, false>(n);
 }
 
 template <int L1, int L2, typename Real> void check_x(int nx) {

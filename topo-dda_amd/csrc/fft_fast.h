@@ -31,6 +31,18 @@ TDDA_HD C tw_at(const C* __restrict__ tw, int j) {
     return w;
 }
 
+// Twiddles w[k] = w_L^(n2 k) (conjugated for DIR > 0), k = 1..K-1, read
+// before the task's first LDS store: the table shares the LDS allocation with
+// the tile, so a table read issued after a tile store waits for that store,
+// and reads interleaved with stores would each pay the full LDS latency.
+template <int K, int DIR, typename C>
+TDDA_HD void load_tw(const C* __restrict__ tw, int n2, C (&w)[K]) {
+    static_for<K>([&](auto i) {
+        constexpr int k = decltype(i)::value;
+        if constexpr (k > 0) w[k] = tw_at<DIR>(tw, n2 * k);
+    });
+}
+
 // Pencils / rows per tile: step-A task counts stay at or below one pass of
 // a 256-thread block (FP64 tiles hold half as many).
 template <typename Real> TDDA_HD constexpr int fast_np_y(int L1, int L2) {
@@ -65,6 +77,8 @@ struct YFast {
         const int b = tau % NP;
         const int n2 = tau / NP;
         if (b >= nb) return;
+        C w[L1];
+        load_tw<L1, DIR>(tw, n2, w);
         C v[L1];
         bool done = false;
         if constexpr (L1 % 2 == 0) {
@@ -88,7 +102,7 @@ struct YFast {
         }
         static_for<L1>([&](auto i) {
             constexpr int k1 = decltype(i)::value;
-            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<DIR>(tw, n2 * k1));
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], w[k1]);
             lds[(k1 * L2 + n2) * NP + b] = v[k1];
         });
     }
@@ -127,11 +141,10 @@ struct YFast {
 // Z pass: forward transform of three components, 3x3 multiply per spectral
 // point, inverse transform; input and output positions < n (pruned).
 // ---------------------------------------------------------------------------
-template <int L1, int L2, int NP, typename C>
+template <int L1, int L2, int NP, typename C, bool PAIR = true>
 struct ZFast {
     static constexpr int L = L1 * L2;
     static constexpr int TA = 3 * NP * L2;
-    static constexpr int TB = NP * L1;
     static constexpr int LDS = 3 * NP * L;
 
     TDDA_HD static int at(int c, int k1, int n2, int b) { return ((c * L + k1 * L2 + n2) * NP) + b; }
@@ -144,6 +157,8 @@ struct ZFast {
         const int n2 = rest % L2;
         const int c = rest / L2;
         if (b >= nb) return;
+        C w[L1];
+        load_tw<L1, -1>(tw, n2, w);
         C v[L1];
         bool done = false;
         if constexpr (L1 % 2 == 0) {
@@ -167,20 +182,31 @@ struct ZFast {
         }
         static_for<L1>([&](auto i) {
             constexpr int k1 = decltype(i)::value;
-            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<-1>(tw, n2 * k1));
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], w[k1]);
             lds[at(c, k1, n2, b)] = v[k1];
         });
     }
 
-    // step B forward + multiply (mul(b, kz, IC<k2>, v0, v1, v2), kz = k1 +
-    // L1 k2) + step B inverse, in place: the task reads and writes the same
-    // LDS entries.
-    template <typename Mul>
-    TDDA_HD static void mid_b(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Mul mul) {
-        const int b = tau % NP;
-        const int k1 = tau / NP;
-        if (b >= nb) return;
-        C v[3][L2];
+    // Step B: forward L2-point DFTs, 3x3 multiply per spectral point,
+    // inverse L2-point DFTs, in place (a task reads and writes the same LDS
+    // entries).  Tasks are (pencil, unit j): unit j pairs k1 = j with
+    // k1 = L1 - j, whose points kz = j + L1 k2 and L - kz (at k2' = L2-1-k2)
+    // are z-mirrors, so one octant record serves both (the z-odd components
+    // 02 and 12 change sign); j = 0 and j = L1/2 stand alone.  Without PAIR
+    // (fewer registers) every k1 is its own task.
+    // fetch(b, kz, C (&g)[6]) gives the components (00,01,02,11,12,22) at
+    // (pencil b, kz) with their parity signs.
+    static constexpr int NU = PAIR ? L1 / 2 + 1 : L1;
+    static constexpr int TB = NP * NU;
+
+    TDDA_HD static void sym_mul(const C (&g)[6], C& v0, C& v1, C& v2) {
+        const C a0 = v0, a1 = v1, a2 = v2;
+        v0 = cadd(cadd(cmul(g[0], a0), cmul(g[1], a1)), cmul(g[2], a2));
+        v1 = cadd(cadd(cmul(g[1], a0), cmul(g[3], a1)), cmul(g[4], a2));
+        v2 = cadd(cadd(cmul(g[2], a0), cmul(g[4], a1)), cmul(g[5], a2));
+    }
+
+    TDDA_HD static void load_fwd(const C* __restrict__ lds, int k1, int b, C (&v)[3][L2]) {
         static_for<3>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             static_for<L2>([&](auto i) {
@@ -189,20 +215,59 @@ struct ZFast {
             });
             SDft<L2, -1, C>::run(v[c]);
         });
-        static_for<L2>([&](auto i) {
-            constexpr int k2 = decltype(i)::value;
-            mul(b, k1 + L1 * k2, i, v[0][k2], v[1][k2], v[2][k2]);
-        });
+    }
+
+    // w: load_tw<L2, 1>(tw, k1, w), read before the task's first store
+    TDDA_HD static void inv_store(C* __restrict__ lds, const C (&w)[L2], int k1, int b, C (&v)[3][L2]) {
         static_for<3>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             SDft<L2, 1, C>::run(v[c]);
             static_for<L2>([&](auto i) {
                 constexpr int n2 = decltype(i)::value;
-                C w = v[c][n2];
-                if constexpr (n2 > 0) w = cmul(w, tw_at<1>(tw, n2 * k1));
-                lds[at(c, k1, n2, b)] = w;
+                C x = v[c][n2];
+                if constexpr (n2 > 0) x = cmul(x, w[n2]);
+                lds[at(c, k1, n2, b)] = x;
             });
         });
+    }
+
+    template <typename Fetch>
+    TDDA_HD static void mid(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Fetch fetch) {
+        const int b = tau % NP;
+        const int j = tau / NP;
+        if (b >= nb) return;
+        if (!PAIR || j == 0 || 2 * j == L1) {
+            C w[L2];
+            load_tw<L2, 1>(tw, j, w);
+            C v[3][L2];
+            load_fwd(lds, j, b, v);
+            static_for<L2>([&](auto i) {
+                constexpr int k2 = decltype(i)::value;
+                C g[6];
+                fetch(b, j + L1 * k2, g);
+                sym_mul(g, v[0][k2], v[1][k2], v[2][k2]);
+            });
+            inv_store(lds, w, j, b, v);
+            return;
+        }
+        C wa[L2], wb[L2];
+        load_tw<L2, 1>(tw, j, wa);
+        load_tw<L2, 1>(tw, L1 - j, wb);
+        C va[3][L2], vb[3][L2];
+        load_fwd(lds, j, b, va);
+        load_fwd(lds, L1 - j, b, vb);
+        static_for<L2>([&](auto i) {
+            constexpr int k2 = decltype(i)::value;
+            constexpr int m2 = L2 - 1 - k2;
+            C g[6];
+            fetch(b, j + L1 * k2, g);
+            sym_mul(g, va[0][k2], va[1][k2], va[2][k2]);
+            g[2] = mk<C>(-g[2].x, -g[2].y);
+            g[4] = mk<C>(-g[4].x, -g[4].y);
+            sym_mul(g, vb[0][m2], vb[1][m2], vb[2][m2]);
+        });
+        inv_store(lds, wa, j, b, va);
+        inv_store(lds, wb, L1 - j, b, vb);
     }
 
     // inverse, step A: store(c, b, pos, value) for pos < n
@@ -259,6 +324,8 @@ struct XFast {
         const int n2 = rest % L2;
         const int r = rest / L2;
         if (r >= nr) return;
+        C w[L1];
+        load_tw<L1, -1>(tw, n2, w);
         C v[L1];
         bool done = false;
         if constexpr (L1 % 2 == 0) {
@@ -283,7 +350,7 @@ struct XFast {
         const int b = 3 * r + c;
         static_for<L1>([&](auto i) {
             constexpr int k1 = decltype(i)::value;
-            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<-1>(tw, n2 * k1));
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], w[k1]);
             lds[at(b, k1, n2)] = v[k1];
         });
     }
@@ -316,6 +383,8 @@ struct XFast {
         if (b >= 3 * nr) return;
         const int r = b / 3;
         const int c = b - 3 * r;
+        C w[L1];
+        load_tw<L1, 1>(tw, n2, w);
         C v[L1];
         static_for<L1>([&](auto i) {
             constexpr int n1 = decltype(i)::value;
@@ -324,7 +393,7 @@ struct XFast {
         SDft<L1, 1, C>::run(v);
         static_for<L1>([&](auto i) {
             constexpr int k1 = decltype(i)::value;
-            if constexpr (k1 > 0) v[k1] = cmul(v[k1], tw_at<1>(tw, n2 * k1));
+            if constexpr (k1 > 0) v[k1] = cmul(v[k1], w[k1]);
             lds[at(b, k1, n2)] = v[k1];
         });
     }

@@ -146,6 +146,11 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
 // ----------------------------------------------------------------------------
 // Z pass: forward z FFT, Green's multiply, inverse z FFT (z < Nz in and out)
 // ----------------------------------------------------------------------------
+// z-mirror pairs in the multiply step (one octant record per pair of
+// spectral points) measured slower than one task per k1: fewer threads then
+// carry the step's loads.
+constexpr bool kZPairs = false;
+
 template <typename Real, int CLS>
 __global__ void __launch_bounds__(FFT_THREADS)
 k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx_t<Real>* __restrict__ twpool,
@@ -160,7 +165,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
         if constexpr (z_class_of<decltype(s2)>() != CLS) return;   // the other class's launch
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_z<Real>(L1, L2);
-        using Z = ZFast<L1, L2, NP, C>;
+        using Z = ZFast<L1, L2, NP, C, kZPairs>;
         C* lds = reinterpret_cast<C*>(smem_zf);
         C* twl = lds + Z::LDS;
         load_twiddles(twl, twpool + J.pz.tw_off, L);
@@ -178,10 +183,10 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
             Z::fwd_a(t, lds, twl, nb, nz, [&](int c, int b, int pos) -> C { return gload<C>(g, c * S + pos * plane + b); });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TB; t += FFT_THREADS)
-            Z::mid_b(t, lds, twl, nb, [&](int b, int kz, auto, C& v0, C& v1, C& v2) {
+            Z::mid(t, lds, twl, nb, [&](int b, int kz, C (&gg)[6]) {
                 const unsigned p = pl0 + b;
                 const unsigned ky = fdiv(p, mlx);
-                green_apply(G, J, (int)(p - ky * lx), (int)ky, kz, v0, v1, v2);
+                green_fetch(G, J, (int)(p - ky * lx), (int)ky, kz, gg);
             });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TA; t += FFT_THREADS)
