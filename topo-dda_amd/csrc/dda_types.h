@@ -35,6 +35,7 @@ struct KScalars {
     int iterations;     // reported iteration count (reference ITERATION)
     int n_ru;           // number of reliable updates done
     int pending;        // 1: P += alpha p + zeta t, r = t - zeta s of the last step not yet applied
+    int dirty;          // refined mode: dP may hold steps not yet added to P
 };
 
 // doubles per tile in the per-job partial-sum buffer
@@ -66,6 +67,10 @@ struct GridJob {
     // Krylov vectors, 3*ndip entries each, Krylov precision (float2 or double2)
     void *r, *r0, *p, *v, *t, *s;
     double2* P;                // solution, FP64
+    // refined mode: c64 sum of the steps since the last reliable update (the
+    // solution is P + dP; reliable updates and the end of a solve fold it
+    // into P); null in FP64 mode
+    void* dP;
     const double2* rhs;        // right-hand side, FP64
     // plain operator application y = A x + al x (tests, gradients)
     const void* xin;

@@ -206,12 +206,28 @@ __device__ inline XfScalars xf_scalars(const GridJob& J) {
     return s;
 }
 
-// the pending step update of one element; returns the new P
+// c64 Krylov vectors: the solution increments go to dP between reliable updates
+template <typename KC>
+constexpr bool kSplitSolution = sizeof(KC) == sizeof(float2);
+
+// P + dP into P (dP cleared); returns the new P
+template <typename KC>
+__device__ inline double2 fold_solution(const GridJob& J, unsigned vi, double2 add) {
+    double2 Pn = cadd(gload<double2>(J.P, vi), add);
+    if constexpr (kSplitSolution<KC>) {
+        Pn = cadd(Pn, to_d(gload<KC>(J.dP, vi)));
+        gstore<KC>(J.dP, vi, mk<KC>(0.0, 0.0));
+    }
+    gstore<double2>(J.P, vi, Pn);
+    return Pn;
+}
+
+// the pending step update of one element, with dP folded into P; returns
+// the new P
 template <typename KC>
 __device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfScalars& sc, double2 pv, bool write_r) {
     const double2 td = to_d(gload<KC>(J.t, vi));
-    const double2 Pn = cadd(gload<double2>(J.P, vi), cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
-    gstore<double2>(J.P, vi, Pn);
+    const double2 Pn = fold_solution<KC>(J, vi, cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
     if (write_r) {
         const double2 sd = to_d(gload<KC>(J.s, vi));
         gstore<KC>(J.r, vi, from_d<KC>(csub(td, dcmul(sc.zeta, sd))));
@@ -230,6 +246,7 @@ __device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc) {
             const double2 pv = to_d(gload<KC>(J.p, i));
             return from_d<C>(apply_pending<KC>(J, i, sc, pv, false));
         }
+        if constexpr (kSplitSolution<KC>) return from_d<C>(fold_solution<KC>(J, i, make_double2(0.0, 0.0)));
         return from_d<C>(gload<double2>(J.P, i));
     } else if (MODE == XF_P) {
         const double2 pv = to_d(gload<KC>(J.p, i));
@@ -237,7 +254,11 @@ __device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc) {
         if (sc.pending) {
             const double2 td = to_d(gload<KC>(J.t, i));
             const double2 sd = to_d(gload<KC>(J.s, i));
-            gstore<double2>(J.P, i, cadd(gload<double2>(J.P, i), cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td))));
+            const double2 step = cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td));
+            if constexpr (kSplitSolution<KC>)
+                gstore<KC>(J.dP, i, from_d<KC>(cadd(to_d(gload<KC>(J.dP, i)), step)));
+            else
+                gstore<double2>(J.P, i, cadd(gload<double2>(J.P, i), step));
             const KC rk = from_d<KC>(csub(td, dcmul(sc.zeta, sd)));
             gstore<KC>(J.r, i, rk);
             rv = to_d(rk);
@@ -333,7 +354,8 @@ __device__ inline void xi_finish(const GridJob& J, int tile, double (&acc)[XI_NA
 //              |r|^2 = |t|^2 - |<s,t>|^2/|s|^2 (zeta minimises |t - zeta s|);
 //              the vector update becomes pending and the step is decided
 //   XI_RESID : true residual of the updated P (the X forward pass applied the
-//              pending update): reliable-update bookkeeping and decision
+//              pending update and folded dP into P): reliable-update
+//              bookkeeping and decision
 template <int MODE>
 __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int* ru_jobs) {
     const double2 dot = make_double2(sum[0], sum[1]);
@@ -349,11 +371,13 @@ __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int
         const double2 rho_new = csub(r0t, dcmul(zeta, r0s));
         const double rn2 = fmax(sum[7] - (dot.x * dot.x + dot.y * dot.y) / ss, 0.0);
         ks->pending = 1;
+        if (ks->refined) ks->dirty = 1;   // the X forward pass adds the step to dP
         step_decide(ks, rho_new, rn2);
         if (ks->need_ru != 0 && ru_jobs != nullptr) atomicAdd(ru_jobs, 1);
     } else if (MODE == XI_RESID) {
         if (ks->need_ru != 0 && ru_jobs != nullptr) atomicSub(ru_jobs, 1);
         ks->pending = 0;
+        ks->dirty = 0;                    // the X forward pass folded dP into P
         ks->need_ru = 0;
         ks->n_ru += 1;
         ks->maxr2 = sum[2];

@@ -285,8 +285,9 @@ k_zpass(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const c
 // ----------------------------------------------------------------------------
 // Pending end-of-step update of jobs that stopped at the XI_S decision
 // (converged, iteration limit or breakdown with the step's P and r update
-// still owed): P += alpha p + zeta t, r = t - zeta s.  k_finalize_clear,
-// launched after it, clears the flag, so a second pair changes nothing.
+// still owed): P += alpha p + zeta t, r = t - zeta s; in refined mode the
+// c64 increments dP are folded into P as well.  k_finalize_clear, launched
+// after it, clears the flags, so a second pair changes nothing.
 // ----------------------------------------------------------------------------
 template <typename KReal>
 __global__ void __launch_bounds__(FFT_THREADS)
@@ -295,7 +296,9 @@ k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     const KScalars* ks = J.ks;
-    if (ks == nullptr || ks->status == JOB_ACTIVE || ks->pending == 0) return;
+    if (ks == nullptr || ks->status == JOB_ACTIVE) return;
+    const bool fold = kSplitSolution<KC> && ks->dirty != 0 && J.dP != nullptr;
+    if (ks->pending == 0 && !fold) return;
     XfScalars sc{ks->alpha, ks->beta, ks->zeta, 1};
     const long n3 = 3L * J.ndip;
     const long base = (long)jt.y * FFT_THREADS * UPD_EPT;
@@ -303,7 +306,9 @@ k_finalize(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap) {
 #pragma unroll
     for (int k = 0; k < UPD_EPT; ++k) {
         const long i = base + (long)k * FFT_THREADS + threadIdx.x;
-        if (i < n3) apply_pending<KC>(J, (unsigned)i, sc, to_d(p[i]), true);
+        if (i >= n3) continue;
+        if (ks->pending != 0) apply_pending<KC>(J, (unsigned)i, sc, to_d(p[i]), true);
+        else fold_solution<KC>(J, (unsigned)i, make_double2(0.0, 0.0));
     }
 }
 
@@ -312,7 +317,10 @@ __global__ void k_finalize_clear(const GridJob* __restrict__ jobs, int njobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= njobs) return;
     KScalars* ks = jobs[j].ks;
-    if (ks != nullptr && ks->status != JOB_ACTIVE) ks->pending = 0;
+    if (ks != nullptr && ks->status != JOB_ACTIVE) {
+        ks->pending = 0;
+        ks->dirty = 0;
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -386,6 +394,7 @@ k_init(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, int warm
             static_cast<KC*>(J.r0)[i] = rk;
             static_cast<KC*>(J.p)[i] = zero;
             static_cast<KC*>(J.v)[i] = zero;
+            if (J.dP != nullptr) static_cast<KC*>(J.dP)[i] = zero;
             const double2 rd = to_d(rk);
             acc[0] += rd.x * rd.x + rd.y * rd.y;
             acc[1] += b.x * b.x + b.y * b.y;
@@ -414,6 +423,7 @@ k_init_reduce(const GridJob* __restrict__ jobs, int warm) {
     ks->iter = -1;
     ks->need_ru = 0;
     ks->pending = 0;
+    ks->dirty = 0;
     ks->n_ru = 0;
     ks->iterations = 0;
     ks->err = 0.0;

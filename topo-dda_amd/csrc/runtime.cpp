@@ -347,7 +347,7 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     const size_t n3 = 3ull * ndip;
     const size_t kbytes = refined ? sizeof(float2) : sizeof(double2);
     grid.alloc(grid_elems(o) * sizeof(double2));
-    kvec.alloc(6 * n3 * kbytes);
+    kvec.alloc((refined ? 7 : 6) * n3 * kbytes);
     P.alloc(n3);
     rhs.alloc(n3);
     scratch64.alloc(n3);
@@ -384,6 +384,7 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.v = k + 3 * n3 * kbytes;
     job.t = k + 4 * n3 * kbytes;
     job.s = k + 5 * n3 * kbytes;
+    job.dP = refined ? static_cast<void*>(k + 6 * n3 * kbytes) : nullptr;
     job.P = P.get();
     job.rhs = rhs.get();
     job.partial = partial.get();

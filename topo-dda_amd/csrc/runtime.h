@@ -164,7 +164,7 @@ class System {
     bool refined;
     GridJob job;   // host copy of the descriptor (ks set by the batch)
     DevBuf<char> grid;          // c128-sized work grid (c64 view aliases it)
-    DevBuf<char> kvec;          // r, r0, p, v, t, s
+    DevBuf<char> kvec;          // r, r0, p, v, t, s (+ dP in refined mode)
     DevBuf<double2> P, rhs, scratch64;
     DevBuf<double2> al64;
     DevBuf<float2> al32;
