@@ -203,6 +203,27 @@ def test_refined_solve_accuracy(dm, O, tables):
     assert err32 < 3 * max(err64, 1e-6)
 
 
+def test_refined_maxit_returns_whole_solution(dm, O, tables):
+    """A refined solve stopped by the iteration limit ends on a reliable
+    update: the reported error is the FP64 true residual of the returned
+    polarization, which therefore holds every step (the c64 increments
+    accumulated since the last update included)."""
+    _log("[gpu] refined maxit 16^3")
+    a = _args(O, tables, (16, 16, 16), method="LDR", seed=3)
+    g = dm.DDAModel(*a)
+    o = O.OracleDDAModel(*a)
+    p0 = np.zeros(3 * o.N, complex)
+    g.setSolverPrecision("fp32_refined")
+    for k in (5, 23):
+        g.solveElectricField(p0, k, 1e-30)
+        info = g.solveInfo(0)
+        assert info["status"] == "maxit" and info["iterations"] == k
+        P = g.getPolarization()
+        r = np.linalg.norm(o.E - o.apply(P)) / np.linalg.norm(o.E)
+        _log(f"[gpu] maxit {k}: reported {info['error']:.6e} oracle {r:.6e} updates {info.get('reliable_updates')}")
+        assert info["error"] == pytest.approx(r, rel=1e-6)
+
+
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 def test_gradients_match_oracle(dm, O, tables, mode):
     _log(f"[gpu] gradients {mode}")
