@@ -174,11 +174,14 @@ def test_residual_history_matches_oracle(dm, O, tables, name, case):
     assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
 
 
-def test_refined_solve_accuracy(dm, O, tables):
+@pytest.mark.parametrize("shape", [(16, 16, 16), (6, 5, 32)], ids=["16cube", "6x5x32"])
+def test_refined_solve_accuracy(dm, O, tables, shape):
     """complex64 transforms/Krylov with FP64 reliable updates: the true FP64
-    residual meets the tolerance and E is close to the FP64 solution."""
-    _log("[gpu] refined solve 16^3")
-    a = _args(O, tables, (16, 16, 16), method="LDR", seed=0)
+    residual meets the tolerance and E is close to the FP64 solution.  The
+    z lengths 32 and 64 take the c64 Z splits 8 x 4 and 16 x 4 (the second
+    with a partial last tile of pencils)."""
+    _log(f"[gpu] refined solve {shape}")
+    a = _args(O, tables, shape, method="LDR", seed=0)
     g = dm.DDAModel(*a)
     o = O.OracleDDAModel(*a)
     p0 = np.zeros(3 * o.N, complex)
