@@ -57,11 +57,25 @@ template <int L1, int L2, typename Real> void check_y(int n_in, int n_out, int d
     rep<C>("Y", L, e / nrm, sizeof(Real) == 4 ? 3e-6 : 1e-14);
 }
 
-template <int L1, int L2, typename Real, bool PAIR> void check_z_impl(int n) {
+template <int L1, int L2, typename Real, bool PAIR, bool XCHG = false> void check_z_impl(int n) {
     using C = cx_t<Real>;
     constexpr int L = L1 * L2;
     constexpr int NP = fast_np_z<Real>(L1, L2);
-    using Z = ZFast<L1, L2, NP, C, PAIR>;
+    using Z = ZFast<L1, L2, NP, C, PAIR, XCHG>;
+    if constexpr (Z::kX) {
+        // lane l + 32 of every wave holds the same pencil and the z-mirror k1
+        std::vector<int> seen(L1, 0);
+        for (int tau = 0; tau < Z::TB; ++tau) {
+            const int b = tau % NP, k1 = Z::xchg_k1(tau / NP);
+            if (b == 0) seen[k1]++;
+            const int pt = tau ^ 32, pk1 = Z::xchg_k1(pt / NP);
+            const bool self = k1 == 0 || 2 * k1 == L1;
+            const bool ok = pt % NP == b && (self ? (pk1 == 0 || 2 * pk1 == L1) && pk1 != k1 : pk1 == L1 - k1);
+            if (!ok) { printf("FAIL Z exchange order L=%d tau=%d\n", L, tau); worst = 1; }
+        }
+        for (int k = 0; k < L1; ++k)
+            if (seen[k] != 1) { printf("FAIL Z exchange k1 cover L=%d k1=%d\n", L, k); worst = 1; }
+    }
     auto tw = table<C>(L);
     int nb = NP - 3 > 0 ? NP - 3 : NP;
     std::vector<cld> X(3 * NP * L);
@@ -81,9 +95,12 @@ template <int L1, int L2, typename Real, bool PAIR> void check_z_impl(int n) {
     std::vector<C> lds(Z::LDS);
     auto idx = [&](int c, int b, int pos) { return (c * NP + b) * L + pos; };
     for (int t = 0; t < Z::TA; ++t) Z::fwd_a(t, lds.data(), tw.data(), nb, n, [&](int c, int b, int pos) { return g[idx(c, b, pos)]; });
-    for (int t = 0; t < Z::TB; ++t) Z::mid(t, lds.data(), tw.data(), nb, [&](int b, int kz, C (&gg)[6]) {
+    auto fetch = [&](int b, int kz, C (&gg)[6]) {
         for (int k = 0; k < 6; ++k) { cld z = G[(k * NP + b) * L + kz]; gg[k] = mk<C>((double)z.real(), (double)z.imag()); }
-    });
+    };
+    // the mirror lane's record for its point kz_m (what the wave exchange delivers)
+    auto exch = [&](int b, int kz_m, const C (&)[6], C (&out)[6]) { fetch(b, kz_m, out); };
+    for (int t = 0; t < Z::TB; ++t) Z::mid(t, lds.data(), tw.data(), nb, fetch, exch);
     for (int t = 0; t < Z::TA; ++t) Z::inv_a(t, lds.data(), nb, n, [&](int c, int b, int pos, C v) { out[idx(c, b, pos)] = v; });
     double e = 0, nrm = 0;
     for (int b = 0; b < nb; ++b) {
@@ -103,12 +120,14 @@ template <int L1, int L2, typename Real, bool PAIR> void check_z_impl(int n) {
         }
     }
     for (int b = nb; b < NP; ++b) for (int c = 0; c < 3; ++c) if (out[idx(c, b, 0)].x != 7) { printf("Z wrote partial\n"); worst = 1; }
-    rep<C>(PAIR ? "Z(pairs)" : "Z", L, e / nrm, sizeof(Real) == 4 ? 5e-6 : 1e-14);
+    rep<C>(XCHG ? "Z(exchange)" : PAIR ? "Z(pairs)" : "Z", L, e / nrm, sizeof(Real) == 4 ? 5e-6 : 1e-14);
 }
-// both multiply-step task layouts: z-mirror pairs and one task per k1
+// the multiply-step task layouts: z-mirror pairs, one task per k1, and one
+// task per k1 in the lane-exchange order
 template <int L1, int L2, typename Real> void check_z(int n) {
     check_z_impl<L1, L2, Real, true>(n);
     check_z_impl<L1, L2, Real, false>(n);
+    check_z_impl<L1, L2, Real, false, true>(n);
 }
 
 template <int L1, int L2, typename Real> void check_x(int nx) {

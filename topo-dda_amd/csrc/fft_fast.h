@@ -141,7 +141,7 @@ struct YFast {
 // Z pass: forward transform of three components, 3x3 multiply per spectral
 // point, inverse transform; input and output positions < n (pruned).
 // ---------------------------------------------------------------------------
-template <int L1, int L2, int NP, typename C, bool PAIR = true>
+template <int L1, int L2, int NP, typename C, bool PAIR = true, bool XCHG = false>
 struct ZFast {
     static constexpr int L = L1 * L2;
     static constexpr int TA = 3 * NP * L2;
@@ -199,6 +199,25 @@ struct ZFast {
     static constexpr int NU = PAIR ? L1 / 2 + 1 : L1;
     static constexpr int TB = NP * NU;
 
+    // Lane exchange (XCHG, one task per k1): the tasks of a 64-wide wave are
+    // ordered so that lane l + 32 holds the z-mirror task k1' = L1 - k1 of
+    // lane l's pencil; each lane loads the octant records of its points
+    // k2 < L2/2 and takes those of k2 >= L2/2 from its mirror lane, whose
+    // own points k2' = L2-1-k2 are their z-mirrors (the records differ only
+    // in the sign of the z-odd components 02 and 12).  k1 = 0 and k1 = L1/2
+    // are their own mirrors (they share one pair of wave halves): they reuse
+    // their own records, and k1 = 0 also loads the one at kz = L/2, issued
+    // with the others.
+    static constexpr bool kX = XCHG && !PAIR && L1 % 2 == 0 && L2 % 2 == 0 && 32 % NP == 0 && TB % 64 == 0;
+    // k1 of task slot u = tau / NP under the exchange order
+    TDDA_HD static int xchg_k1(int u) {
+        constexpr int h = 32 / NP;                  // mirror pairs per wave
+        const int gp = (u / (2 * h)) * h + u % h;   // pair index; the last is (0, L1/2)
+        const bool hi = (u % (2 * h)) >= h;
+        if (gp == L1 / 2 - 1) return hi ? L1 / 2 : 0;
+        return hi ? L1 - (gp + 1) : gp + 1;
+    }
+
     TDDA_HD static void sym_mul(const C (&g)[6], C& v0, C& v1, C& v2) {
         const C a0 = v0, a1 = v1, a2 = v2;
         v0 = cadd(cadd(cmul(g[0], a0), cmul(g[1], a1)), cmul(g[2], a2));
@@ -231,8 +250,65 @@ struct ZFast {
         });
     }
 
-    template <typename Fetch>
-    TDDA_HD static void mid(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Fetch fetch) {
+    // exch(b, kz_m, mine, out): out = the record the mirror lane loaded for
+    // its point kz_m = L - kz, given this lane's record `mine` for the same
+    // instruction (called by every lane of the wave alike; only when kX)
+    TDDA_HD static int uniform_wave(int tau) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_readfirstlane(tau >> 6);
+#else
+        return tau >> 6;
+#endif
+    }
+
+    template <typename Fetch, typename Exch>
+    TDDA_HD static void mid(int tau, C* __restrict__ lds, const C* __restrict__ tw, int nb, Fetch fetch, Exch exch) {
+        if constexpr (kX) {
+            const int b = tau % NP;
+            const int j = xchg_k1(tau / NP);
+            if (b >= nb) return;
+            const bool self = j == 0 || 2 * j == L1;
+            C w[L2];
+            load_tw<L2, 1>(tw, j, w);
+            C own[L2 / 2][6], half[6];
+            static_for<L2 / 2>([&](auto i) {
+                constexpr int k2 = decltype(i)::value;
+                fetch(b, j + L1 * k2, own[k2]);
+            });
+            // kz = L/2 is its own mirror: k1 = 0 loads that record too; the
+            // self pair sits in the last wave, which loads it for all its lanes
+            // (a wave-uniform branch)
+            if (uniform_wave(tau) == TB / 64 - 1) fetch(b, L / 2, half);
+            C v[3][L2];
+            load_fwd(lds, j, b, v);
+            static_for<L2>([&](auto i) {
+                constexpr int k2 = decltype(i)::value;
+                C g[6];
+                if constexpr (k2 < L2 / 2) {
+                    static_for<6>([&](auto q) { g[decltype(q)::value] = own[k2][decltype(q)::value]; });
+                } else {
+                    C ex[6];
+                    exch(b, L - (j + L1 * k2), own[L2 - 1 - k2], ex);
+                    // the z-mirror record: the mirror lane's, or for k1 = L1/2
+                    // (kz' at k2' = L2-1-k2) and k1 = 0 (kz' at k2' = L2-k2)
+                    // this lane's own
+                    static_for<6>([&](auto q) {
+                        constexpr int c = decltype(q)::value;
+                        C m;
+                        if constexpr (2 * k2 == L2)
+                            m = self ? own[L2 - 1 - k2][c] : ex[c];
+                        else
+                            m = !self ? ex[c] : (j != 0 ? own[L2 - 1 - k2][c] : own[L2 - k2][c]);
+                        if constexpr (c == 2 || c == 4) m = mk<C>(-m.x, -m.y);
+                        if constexpr (2 * k2 == L2) g[c] = j == 0 ? half[c] : m;
+                        else g[c] = m;
+                    });
+                }
+                sym_mul(g, v[0][k2], v[1][k2], v[2][k2]);
+            });
+            inv_store(lds, w, j, b, v);
+            return;
+        }
         const int b = tau % NP;
         const int j = tau / NP;
         if (b >= nb) return;

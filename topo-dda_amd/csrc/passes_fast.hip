@@ -148,7 +148,8 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
 // ----------------------------------------------------------------------------
 // z-mirror pairs in the multiply step (one octant record per pair of
 // spectral points) measured slower than one task per k1: fewer threads then
-// carry the step's loads.
+// carry the step's loads.  The c64 kernel of the L2 = 4 splits shares the
+// records between mirror lanes of a wave instead (ZFast XCHG).
 constexpr bool kZPairs = false;
 
 template <typename Real, int CLS>
@@ -165,7 +166,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
         if constexpr (z_class_of<decltype(s2)>() != CLS) return;   // the other class's launch
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_z<Real>(L1, L2);
-        using Z = ZFast<L1, L2, NP, C, kZPairs>;
+        using Z = ZFast<L1, L2, NP, C, kZPairs, sizeof(Real) == 4 && L2 == 4>;
         C* lds = reinterpret_cast<C*>(smem_zf);
         C* twl = lds + Z::LDS;
         load_twiddles(twl, twpool + J.pz.tw_off, L);
@@ -183,11 +184,20 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
             Z::fwd_a(t, lds, twl, nb, nz, [&](int c, int b, int pos) -> C { return gload<C>(g, c * S + pos * plane + b); });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TB; t += FFT_THREADS)
-            Z::mid(t, lds, twl, nb, [&](int b, int kz, C (&gg)[6]) {
-                const unsigned p = pl0 + b;
-                const unsigned ky = fdiv(p, mlx);
-                green_fetch(G, J, (int)(p - ky * lx), (int)ky, kz, gg);
-            });
+            Z::mid(
+                t, lds, twl, nb,
+                [&](int b, int kz, C (&gg)[6]) {
+                    const unsigned p = pl0 + b;
+                    const unsigned ky = fdiv(p, mlx);
+                    green_fetch(G, J, (int)(p - ky * lx), (int)ky, kz, gg);
+                },
+                [&](int, int, const C (&mine)[6], C (&out)[6]) {
+                    static_for<6>([&](auto q) {
+                        constexpr int k = decltype(q)::value;
+                        out[k].x = __shfl_xor(mine[k].x, 32);
+                        out[k].y = __shfl_xor(mine[k].y, 32);
+                    });
+                });
         __syncthreads();
         for (int t = threadIdx.x; t < Z::TA; t += FFT_THREADS)
             Z::inv_a(t, lds, nb, nz, [&](int c, int b, int pos, C v) { gstore<C>(g, c * S + pos * plane + b, v); });
