@@ -48,8 +48,10 @@ TDDA_HD void load_tw(const C* __restrict__ tw, int n2, C (&w)[K]) {
 template <typename Real> TDDA_HD constexpr int fast_np_y(int L1, int L2) {
     return (256 / L2) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L1;
 }
+// (c64, L1 = 12: 16 pencils rather than 21, so that the Z multiply step can
+// pair z-mirror tasks 32 lanes apart; 64 of the 256 threads idle)
 template <typename Real> TDDA_HD constexpr int fast_np_z(int L1, int L2) {
-    return (256 / L1) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L2;
+    return (L1 == 12 && sizeof(Real) == 4 ? 16 : (256 / L1) / (sizeof(Real) == 8 ? 2 : 1)) + 0 * L2;
 }
 template <typename Real> TDDA_HD constexpr int fast_nr_x(int L1, int L2) {
     return (256 / (3 * L2)) / (sizeof(Real) == 8 ? 2 : 1) + 0 * L1;
