@@ -83,6 +83,7 @@ typedef struct {
     double error;    /* final ||r|| / ||rhs|| (recursive residual in FP64 mode, true residual in refined mode) */
     int reliable_updates;
     double ms;       /* wall time of the solve */
+    double err_est;  /* refined mode: estimated relative error of E = al P 3/(eps+2) (0: none) */
 } tdda_solve_info;
 
 /* last failure message of this thread ("" if none) */
@@ -188,6 +189,12 @@ int tdda_problem_objective(tdda_problem* p, int kind, int power, const double* t
  * host or device.  value (nullable): the objective; adj (nullable): adjoint solve record. */
 int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* targets, int ntargets, int maxit,
                           double tol, double* grad, int on_device, double* value, tdda_solve_info* adj);
+/* y = (diag(al) + A) x for all n problems in one batched launch per pass
+ * (AProductCore::Aproduct + DDAModel::Aproductwithalb, src/AProductCore.cpp:461-515,
+ * src/DDAModel.cpp:1194-1203), with c64 transforms (c64 != 0; problems of
+ * precision TDDA_PREC_FP32_REFINED) or FP64 ones.  x, y: host arrays of the
+ * problems' vectors back to back, 3*nx*ny*nz complex each. */
+int tdda_problems_apply(tdda_problem** ps, int n, int c64, const double* x, double* y);
 /* Average ms per launch of the five matvec passes over all n problems in one
  * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
  * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */
@@ -207,6 +214,7 @@ typedef struct {
     int max_slots;       /* requests in flight */
     int chunk;           /* iterations between host checks */
     int keep_fields;     /* keep E of finished requests for tdda_server_take_field */
+    double err_tol;      /* refined: relative field-error target; 0 = tol, < 0 = true residual only */
 } tdda_server_opts;
 
 typedef struct {
@@ -217,6 +225,7 @@ typedef struct {
     double error;
     double t_submit, t_admit, t_done;  /* seconds since the server started */
     int operator_cached;
+    double err_est;      /* refined: estimated relative field error */
 } tdda_result;
 
 typedef struct {
@@ -230,6 +239,8 @@ int tdda_server_create(const tdda_server_opts* o, const double* si, const double
 int tdda_server_destroy(tdda_server* s);
 int tdda_server_submit(tdda_server* s, long id, const tdda_problem_desc* d, const double* rho, long n);
 int tdda_server_poll(tdda_server* s, tdda_result* out, int max, int* n_out);
+/* Moves the field of a finished request out (3*nx*ny*nz complex); a buffer of
+ * the wrong size is refused and the field stays. */
 int tdda_server_take_field(tdda_server* s, long id, double* E, long n);
 int tdda_server_idle(tdda_server* s, int* idle);
 int tdda_server_get_stats(tdda_server* s, tdda_server_stats* out);

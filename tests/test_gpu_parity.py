@@ -115,6 +115,15 @@ def test_golden_wrapper_objective(dm, golden_inputs):
 
 
 def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
+    """FP64 path (the reference's recursive-residual stopping rule) on the
+    golden 22x22x10 FCD case against a tightly converged oracle (1e-11).
+
+    Two solves that stop at a relative residual tol differ from the exact
+    solution by about amp * tol in E, amp = 1.5 .. 3 for these systems (the
+    field error over the residual, measured along whole trajectories).  So
+    the field bound 1e-5 is asserted at tol = 2e-6, the reference's own rule
+    at tol = 1e-5 is checked for its iteration window and its field error is
+    logged, and at tol = 1e-11 both solve the same system to 1e-9."""
     _log("[gpu] golden field vs oracle")
     diel, _ = golden_inputs
     a = O.wrapper_args([22, 22, 10], 15.0, diel, [1.01 + 0j, 5.96282 + 3.80423e-7j], [0, 0, 1], [1, 0, 0], 542,
@@ -122,23 +131,31 @@ def test_golden_field_matches_oracle(dm, O, tables, golden_inputs):
     g = dm.DDAModel(*a)
     o = O.OracleDDAModel(*a, workers=4)
     p0 = np.zeros(3 * 4840, complex)
+    o.solveElectricField(p0, 100000, 1e-11)
+    e_tight = o.getElectricField()
+    obj_tight = o.calculateObjective()
     g.solveElectricField(p0, 100000, 1e-5)
-    o.solveElectricField(p0, 100000, 1e-5)
     info = g.solveInfo(0)
-    _log(f"[gpu] iterations gpu {info['iterations']} oracle {o.info.iterations}")
+    err_ref_rule = _rel(g.getElectricField(), e_tight)
+    o5 = O.OracleDDAModel(*a, workers=4)
+    o5.solveElectricField(p0, 100000, 1e-5)
+    _log(f"[gpu] tol 1e-5: iterations gpu {info['iterations']} oracle {o5.info.iterations}; "
+         f"E err vs tight: gpu {err_ref_rule:.3e} oracle {_rel(o5.getElectricField(), e_tight):.3e}")
     # the oracle alone stops anywhere in 40..46 iterations here when only the
-    # FFT grid (hence the rounding) changes, so the count is not a parity
-    # quantity (the recurrence is pinned in test_residual_history_matches_oracle).
-    # Two solves that each stop at a 1e-5 residual differ by about the error
-    # of either; the tight comparison below is the parity statement.
+    # FFT grid (hence the rounding) changes; the recurrence is pinned step by
+    # step in test_residual_history_matches_oracle
     assert info["status"] == "converged" and info["error"] <= 1e-5
-    assert 30 <= info["iterations"] <= 60
-    assert _rel(g.getElectricField(), o.getElectricField()) < 5e-5
-    assert g.calculateObjective() == pytest.approx(o.calculateObjective(), rel=1e-5)
+    assert 36 <= info["iterations"] <= 50
+    assert err_ref_rule < 5e-5
+    # field within 1e-5 of the tight oracle at the tolerance that achieves it
+    g.solveElectricField(p0, 100000, 2e-6)
+    err = _rel(g.getElectricField(), e_tight)
+    _log(f"[gpu] tol 2e-6: iterations {g.solveInfo(0)['iterations']}, E err vs tight {err:.3e}")
+    assert err <= 1e-5
     # tightly converged: both solve the same system
     g.solveElectricField(p0, 100000, 1e-11)
-    o.solveElectricField(p0, 100000, 1e-11)
-    assert _rel(g.getElectricField(), o.getElectricField()) < 1e-9
+    assert _rel(g.getElectricField(), e_tight) < 1e-9
+    assert g.calculateObjective() == pytest.approx(obj_tight, rel=1e-9)
 
 
 @pytest.mark.parametrize("name,case", [CASES[1], CASES[2], CASES[3]], ids=[CASES[i][0] for i in (1, 2, 3)])
@@ -174,36 +191,50 @@ def test_residual_history_matches_oracle(dm, O, tables, name, case):
     assert _rel(g.getElectricField(), o.getElectricField()) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (6, 5, 32)], ids=["16cube", "6x5x32"])
-def test_refined_solve_accuracy(dm, O, tables, shape):
-    """complex64 transforms/Krylov with FP64 reliable updates: the true FP64
-    residual meets the tolerance and E is close to the FP64 solution.  The
-    z lengths 32 and 64 take the c64 Z splits 8 x 4 and 16 x 4 (the second
+REFINED_CASES = [
+    ("ldr-16cube", dict(shape=(16, 16, 16), method="LDR", seed=0)),
+    ("ldr-6x5x32", dict(shape=(6, 5, 32), method="LDR", seed=0)),
+    ("ldr-24x20x12-oblique", dict(shape=(24, 20, 12), method="LDR", seed=4, lam=450.0, k=(0.6, 0, 0.8),
+                                  e=(0.8, 0, -0.6))),
+    ("fcd-periodic-12x10x6", dict(shape=(12, 10, 6), method="FCD", maxmn=2, seed=1)),
+]
+
+
+@pytest.mark.parametrize("name,case", REFINED_CASES, ids=[c[0] for c in REFINED_CASES])
+def test_refined_solve_accuracy(dm, O, tables, name, case):
+    """complex64 transforms/Krylov with FP64 reliable updates at tol 1e-5:
+    the FP64 true residual of the returned P meets 1e-5 and E is within 1e-5
+    of the tightly converged oracle (1e-12).  The solver stops when its own
+    estimate of the field error (amplification measured between reliable
+    updates, times the true residual, times a safety factor 1.5) meets tol;
+    the estimate is reported and checked against the actual error.  The z
+    lengths 32 and 64 take the c64 Z splits 8 x 4 and 16 x 4 (the second
     with a partial last tile of pencils)."""
-    _log(f"[gpu] refined solve {shape}")
-    a = _args(O, tables, shape, method="LDR", seed=0)
+    _log(f"[gpu] refined solve {name}")
+    a = _args(O, tables, **case)
     g = dm.DDAModel(*a)
     o = O.OracleDDAModel(*a)
-    p0 = np.zeros(3 * o.N, complex)
+    n = 3 * o.N
+    p0 = np.zeros(n, complex)
     o.solveElectricField(p0, 100000, 1e-12)
     e_ref = o.getElectricField()
     g.setSolverPrecision("fp32_refined")
     g.solveElectricField(p0, 100000, 1e-5)
     info = g.solveInfo(0)
-    e32 = g.getElectricField()
+    err32 = _rel(g.getElectricField(), e_ref)
+    r = np.linalg.norm(o.E - o.apply(g.getPolarization())) / np.linalg.norm(o.E)
     g.setSolverPrecision("fp64")
     g.solveElectricField(p0, 100000, 1e-5)
-    e64 = g.getElectricField()
-    err32, err64 = _rel(e32, e_ref), _rel(e64, e_ref)
-    _log(f"[gpu] refined info {info}; E err refined {err32:.3e} fp64 {err64:.3e}")
+    info64 = g.solveInfo(0)
+    err64 = _rel(g.getElectricField(), e_ref)
+    _log(f"[gpu] refined {info}; E err refined {err32:.3e} (estimate {info['err_est']:.3e}), true residual "
+         f"{r:.3e}; fp64 at tol 1e-5: {info64['iterations']} iterations, E err {err64:.3e}")
     assert info["status"] == "converged" and info["error"] <= 1e-5
-    # the true residual of the returned P, recomputed by the oracle in FP64
-    g.setSolverPrecision("fp32_refined")
-    g.solveElectricField(p0, 100000, 1e-5)
-    P = g.getPolarization()
-    r = o.E - o.apply(P)
-    assert np.linalg.norm(r) / np.linalg.norm(o.E) <= 1.0001e-5
-    assert err32 < 3 * max(err64, 1e-6)
+    assert r <= 1.0001e-5
+    assert err32 <= 1e-5
+    assert 1.5 * info["err_est"] <= 1e-5
+    # the estimate follows the actual error
+    assert err32 / 3.0 <= info["err_est"] <= 3.0 * err32
 
 
 def test_refined_maxit_returns_whole_solution(dm, O, tables):
@@ -287,3 +318,78 @@ def test_against_committed_fixture(dm, O, tables):
     assert _rel(g.applyOperator(data["x"]), data["y_fcd"]) < 1e-11
     g.solveElectricField(np.zeros(360, complex), 100000, 1e-11)
     assert _rel(g.getElectricField(), data["e_fcd"]) < 1e-8
+
+
+def test_objective_region_quirk_on_gpu(dm, O, tables):
+    """IntegratedE with integral bounds that make the region a proper subset:
+    `xMin <= g <= xMax` parses as `(xMin <= g) <= xMax` (ObjDDAModel.cpp:135),
+    so xMax = 0 keeps the dipoles with g_x < xMin, and xMax >= 1 keeps all of
+    them.  Objective and both gradient modes against the oracle."""
+    _log("[gpu] objective region quirk")
+    shape = (8, 6, 3)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(9)
+    diel = np.repeat(rng.uniform(0.0, 1.0, n), 3)
+    a = O.wrapper_args(list(shape), 15.0, diel, [1.0 + 0j, 5.96 + 3.89j], [0, 0, 1], [1, 0, 0], 542.0,
+                       symmetry_method="None", apc_max_m=0, apc_max_n=0, apc_method_name="LDR", tables=tables,
+                       integral_xbounds=[3.0, 0.0], integral_ybounds=[0.0, 5.0], integral_zbounds=[2.0, 0.0])
+    o = O.OracleDDAModel(*a)
+    region = o.region
+    gx = o.geometry.reshape(-1, 3)
+    assert np.array_equal(region, (gx[:, 0] < 3) & (gx[:, 2] < 2))
+    assert 0 < region.sum() < n
+    g = dm.DDAModel(*a)
+    p0 = np.zeros(3 * n, complex)
+    g.solveElectricField(p0, 100000, 1e-12)
+    o.solveElectricField(p0, 100000, 1e-12)
+    obj = g.calculateObjective()
+    assert obj == pytest.approx(o.calculateObjective(), rel=1e-10)
+    # the whole-box objective differs: the region matters
+    assert abs(obj - O.objective_integrated(o.al, o.P, np.ones(n, bool), o.pownum)) > 1e-3
+    g.setGradientMode("exact")
+    grad = g.calculateGradients(1e-3, obj, 100000, 1e-12)
+    assert _rel(grad, o.exact_gradient(tol=1e-12)) < 1e-8
+
+
+C64_SHAPES = [(40, 48, 16), (48, 64, 24), (64, 40, 32)]
+
+
+def test_c64_apply_matches_fp64(dm, O, tables):
+    """The c64 transform kernels (length-specialised X/Y passes for 80, 96
+    and 128, the Z pass with its lane-exchanged octant records for 32, 48
+    and 64) against the FP64 transforms of the same operators, all in one
+    batched launch per pass; the FP64 apply itself is checked against the
+    oracle for one of them.  Refined solves of the same shapes then take
+    about as many iterations as FP64 ones: a wrong record in the c64 Z pass
+    would show up as a large apply error or as extra iterations."""
+    from dda_model import native
+    _log("[gpu] c64 apply vs fp64 " + str(C64_SHAPES))
+    rng = np.random.default_rng(12)
+    probs, reqs = [], []
+    for i, s in enumerate(C64_SHAPES):
+        req = native.Request(shape=s, d=15.0, lam=480.0 + 70.0 * i, rho=rng.uniform(0, 1, int(np.prod(s))),
+                             n_k=(0.0, 0.0, 1.0), n_e0=(1.0, 0.0, 0.0))
+        reqs.append(req)
+        probs.append(native.Problem(req, precision="fp32_refined"))
+    xs = [rng.standard_normal(3 * p.n) + 1j * rng.standard_normal(3 * p.n) for p in probs]
+    y64 = native.apply(probs, xs, c64=False)
+    y32 = native.apply(probs, xs, c64=True)
+    for s, a, b in zip(C64_SHAPES, y32, y64):
+        e = _rel(a, b)
+        _log(f"[gpu] {s}: c64 vs fp64 apply {e:.3e}")
+        assert e < 2e-6
+    box = O.OracleFullBox(C64_SHAPES[2], 15.0, reqs[2].lam, reqs[2].rho, workers=8)
+    assert _rel(y64[2], box.apply(xs[2])) < 1e-11
+    # iteration counts of refined vs FP64 solves of the same requests: the
+    # refined solve stops later (on its field-error estimate, which needs a
+    # true residual of about tol / (1.5 amp), amp ~ 2) and its c64 vectors
+    # cost about a tenth more steps; a wrong octant record would cost far more
+    its, infos = {}, {}
+    for prec in ("fp64", "fp32_refined"):
+        ps = [native.Problem(r, precision=prec) for r in reqs]
+        infos[prec] = native.solve(ps, tol=1e-5)
+        its[prec] = [i["iterations"] for i in infos[prec]]
+    _log(f"[gpu] iterations fp64 {its['fp64']} refined {its['fp32_refined']}; refined true residual "
+         f"{[round(i['error'], 8) for i in infos['fp32_refined']]}")
+    for a, b in zip(its["fp32_refined"], its["fp64"]):
+        assert 0.8 * b <= a <= 1.5 * b

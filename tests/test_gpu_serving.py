@@ -1,8 +1,9 @@
 """GPU tests of device-resident problems, batched solves and the request server.
 
-Bar: FP64 fields against the oracle to 1e-8 (tight solves); a request solved
-in a heterogeneous batch or in the server mix gives the same field as the
-same request solved alone (bitwise: per-request tiling and fixed-order
+Bar: FP64 fields against the oracle to 1e-8 (tight solves); refined served
+fields within 1e-5 of the tightly converged oracle; a request solved in a
+heterogeneous batch or in the server mix gives the same field as the same
+request solved alone (bitwise: per-request tiling and fixed-order
 reductions make the arithmetic independent of the mix; the assertion allows
 1e-14 relative).
 """
@@ -117,9 +118,16 @@ def test_server_mix_equals_alone_and_oracle(native, tables):
         s1.close()
         assert got[i]["status"] == "converged" and res[0]["iterations"] == got[i]["iterations"]
         assert _rel(mix[i], alone) <= 1e-14
-    # accuracy of a served result against the oracle
-    e_ref = _oracle_field(O, reqs[0], tables)
-    assert _rel(mix[0], e_ref) < 1e-4
+    # accuracy of served results against the tightly converged oracle: the
+    # refined server stops when its field-error estimate meets tol (1e-5)
+    for i in (0, 4):
+        e_ref = _oracle_field(O, reqs[i], tables)
+        err = _rel(mix[i], e_ref)
+        _log(f"[gpu] served request {i}: E err {err:.3e}, estimate {got[i]['err_est']:.3e}, "
+             f"true residual {got[i]['error']:.3e}")
+        assert got[i]["error"] <= 1e-5
+        assert err <= 1e-5
+        assert err / 3.0 <= got[i]["err_est"] <= 3.0 * err
 
 
 def test_server_admission_by_memory(native, tables):

@@ -198,6 +198,7 @@ int tdda_model_solve_info(tdda_model* m, int which, tdda_solve_info* out) {
         out->error = r.err;
         out->reliable_updates = r.n_ru;
         out->ms = r.ms;
+        out->err_est = r.err_est;
     });
 }
 
@@ -238,6 +239,7 @@ int tdda_model_solve_resident(tdda_model* m, int maxit, double tol, tdda_solve_i
             out->error = r.err;
             out->reliable_updates = r.n_ru;
             out->ms = r.ms;
+            out->err_est = r.err_est;
         }
     });
 }
@@ -334,6 +336,7 @@ int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_so
         o.refined = refined;
         o.maxit = maxit;
         o.tol = tol;
+        o.err_tol = refined ? tol : 0.0;
         std::vector<tdda::SolveResult> r = tdda::solve_batch(sys, o, tdda::default_stream());
         if (infos)
             for (int i = 0; i < n; ++i) {
@@ -342,6 +345,7 @@ int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_so
                 infos[i].error = r[i].err;
                 infos[i].reliable_updates = r[i].n_ru;
                 infos[i].ms = r[i].ms;
+                infos[i].err_est = r[i].err_est;
             }
     });
 }
@@ -405,7 +409,21 @@ int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* ta
             adj->error = r.err;
             adj->reliable_updates = r.n_ru;
             adj->ms = r.ms;
+            adj->err_est = r.err_est;
         }
+    });
+}
+
+int tdda_problems_apply(tdda_problem** ps, int n, int c64, const double* x, double* y) {
+    return guarded([&] {
+        TDDA_REQUIRE(ps != nullptr && n >= 1 && x != nullptr && y != nullptr, tdda::ERR_INVALID, "null argument");
+        std::vector<tdda::System*> sys;
+        for (int i = 0; i < n; ++i) {
+            TDDA_REQUIRE(ps[i] && ps[i]->p, tdda::ERR_INVALID, "null problem");
+            sys.push_back(ps[i]->p->sys.get());
+        }
+        tdda::apply_batch(sys, c64 != 0, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y),
+                          tdda::default_stream());
     });
 }
 
@@ -442,6 +460,7 @@ int tdda_server_create(const tdda_server_opts* o, const double* si, const double
         so.maxit = o->maxit;
         so.tol = o->tol;
         so.ru_delta = o->ru_delta > 0 ? o->ru_delta : 0.1;
+        so.err_tol = o->err_tol > 0 ? o->err_tol : (o->err_tol == 0 ? o->tol : 0.0);
         so.hbm_budget = (size_t)(o->hbm_budget_gb * (double)(1ull << 30));
         so.max_slots = o->max_slots;
         so.chunk = o->chunk;
@@ -488,6 +507,7 @@ int tdda_server_poll(tdda_server* s, tdda_result* out, int max, int* n_out) {
             out[i].t_admit = v[i].t_admit;
             out[i].t_done = v[i].t_done;
             out[i].operator_cached = v[i].op_hit;
+            out[i].err_est = v[i].err_est;
         }
         *n_out = k;
     });
@@ -496,9 +516,11 @@ int tdda_server_poll(tdda_server* s, tdda_result* out, int max, int* n_out) {
 int tdda_server_take_field(tdda_server* s, long id, double* E, long n) {
     return guarded([&] {
         TDDA_REQUIRE(s && s->s && E, tdda::ERR_INVALID, "null argument");
+        TDDA_REQUIRE(n >= 0, tdda::ERR_INVALID, "negative field length");
         std::vector<double2> v;
-        TDDA_REQUIRE(s->s->take_field(id, v), tdda::ERR_STATE, "no field kept for this request id");
-        TDDA_REQUIRE((long)v.size() == n, tdda::ERR_INVALID, "field buffer must hold 3*nx*ny*nz complex");
+        const int rc = s->s->take_field(id, (size_t)n, v);
+        TDDA_REQUIRE(rc != 1, tdda::ERR_STATE, "no field kept for this request id");
+        TDDA_REQUIRE(rc != 2, tdda::ERR_INVALID, "field buffer must hold 3*nx*ny*nz complex (the field is kept)");
         std::memcpy(E, v.data(), v.size() * sizeof(double2));
     });
 }

@@ -36,10 +36,32 @@ struct KScalars {
     int n_ru;           // number of reliable updates done
     int pending;        // 1: P += alpha p + zeta t, r = t - zeta s of the last step not yet applied
     int dirty;          // refined mode: dP may hold steps not yet added to P
+    // Refined mode, field-error control.  The relative error of the local
+    // field E = f P (f = al 3/(eps + 2) per dipole) behaves as amp * ||r||/||rhs||
+    // over a solve, with amp a property of the system; amp is measured from
+    // the change of E between two reliable updates whose true residuals are
+    // well apart: ||dE||/||E|| ~ amp (r_prev - r_now).  A solve converges when
+    // the true residual meets tol and safety * amp * residual meets tol_e.
+    double tol_e;       // relative field-error target (0: residual only)
+    double res_target;  // relative residual that triggers the next reliable update at convergence
+    double amp;         // current amplification estimate
+    double prev_true;   // relative true residual of the last reliable update (0: none yet)
+    double err_est;     // estimated relative field error of the returned solution
 };
+
+// prior amplification before any measurement, and the safety factor on the
+// estimate (measured amplifications of the DDA systems are 1.5 to 3, and the
+// estimate tracks the field error to about 25 %)
+constexpr double kAmpPrior = 3.0;
+constexpr double kErrSafety = 1.5;
+// a measurement needs the true residual to have fallen by this factor since
+// the previous reliable update
+constexpr double kAmpSeparation = 0.5;
 
 // doubles per tile in the per-job partial-sum buffer
 constexpr int PARTIAL_STRIDE = 8;
+// doubles per X forward tile of a reliable update: ||f dP||^2, ||f P||^2
+constexpr int XF_NACC = 2;
 
 // One system (diag(al) + A) P = rhs on an Nx x Ny x Nz box.
 struct GridJob {
@@ -77,6 +99,10 @@ struct GridJob {
     void* yout;
     KScalars* ks;
     double* partial;           // [max tiles][PARTIAL_STRIDE]
+    // refined mode: partial sums of the reliable update's X forward pass
+    // (||f dP||^2, ||f P||^2 per tile), apart from the X inverse partials
+    double* partial_xf;        // [X forward tiles][2]
+    const float2* efac;        // [ndip] local-field factor f = al 3/(eps + 2); null: f = 1
 };
 
 }  // namespace tdda

@@ -181,7 +181,18 @@ void Model::update_alpha() {
         al_[i] = ldr_inverse_alpha(K_, cfg_.d, diel_[i], cfg_.ne0, cfg_.nk);
     }
     if (sys64_) sys64_->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
-    if (sys32_) sys32_->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
+    if (sys32_) {
+        sys32_->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
+        upload_field_factor(*sys32_);
+    }
+}
+
+// E = f P with f = al 3/(eps + 2) (DDAModel.cpp:1185-1192): the weight of the
+// refined solver's field-error estimate
+void Model::upload_field_factor(System& s) {
+    std::vector<cd> f(N_);
+    for (int i = 0; i < N_; ++i) f[i] = al_[i] / ((2.0 + diel_[i]) / 3.0);
+    s.set_field_factor(reinterpret_cast<const double2*>(f.data()), default_stream());
 }
 
 // DDAModel.cpp:747-772
@@ -214,6 +225,7 @@ System& Model::system() {
         s.reset(new System(op_, N_, identity_map_ ? nullptr : box_map_.data(), precision_ != PREC_FP64,
                            default_stream()));
         s->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
+        if (s->refined) upload_field_factor(*s);
     }
     return *s;
 }
@@ -236,6 +248,7 @@ SolveResult Model::run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit
     o.refined = precision_ == PREC_FP32_REFINED;
     o.maxit = maxit;
     o.tol = tol;
+    o.err_tol = o.refined ? tol : 0.0;   // refined solves also bound the estimated field error by tol
     o.warm = warm;
     std::vector<System*> v{&s};
     SolveResult r = solve_batch(v, o, st)[0];
@@ -372,6 +385,7 @@ SolveResult Model::solve_resident(int maxit, double tol) {
     o.refined = precision_ == PREC_FP32_REFINED;
     o.maxit = maxit;
     o.tol = tol;
+    o.err_tol = o.refined ? tol : 0.0;
     std::vector<System*> v{&s};
     return solve_batch(v, o, st)[0];
 }

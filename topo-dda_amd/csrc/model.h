@@ -82,6 +82,7 @@ class Model {
 
    private:
     void update_alpha();
+    void upload_field_factor(System& s);
     cd eps_of(double x) const;
     SolveResult run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit, double tol, std::vector<cd>& P);
     System& system();

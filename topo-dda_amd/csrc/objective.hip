@@ -36,7 +36,11 @@ __device__ inline double2 eps_of(const ObjParams& P, double x) {
 }
 
 // d(1/alpha)/d rho = [-(4 pi/d^3)/(eps-1)^2 + K^2 (b2 + b3 S)/d] (m1 - m0)
+// inside [0, 1]; the density is clipped to [0, 1] (eps_of), so the objective
+// is flat outside and the derivative there is zero (at the ends 0 and 1 the
+// one-sided derivative from inside is returned)
 __device__ inline double2 dal_drho(const ObjParams& P, double x) {
+    if (x < 0.0 || x > 1.0) return make_double2(0.0, 0.0);
     const double b2 = 0.1648469, b3 = -1.7700004;
     const double2 e = eps_of(P, x);
     const double2 em1 = make_double2(e.x - 1.0, e.y);
@@ -275,6 +279,7 @@ SolveResult objective_gradient(Problem& p, const ObjectiveSpec& o, int maxit, do
     so.refined = s.refined;
     so.maxit = maxit;
     so.tol = tol;
+    so.err_tol = s.refined ? tol : 0.0;
     std::vector<System*> v{&s};
     SolveResult r = solve_batch(v, so, st)[0];
     DevBuf<double> g;

@@ -38,6 +38,7 @@ template <> struct Tile<float> {
     __device__ static int by(const GridJob& J) { return J.by32; }
     __device__ static int bz(const GridJob& J) { return J.bz32; }
     __device__ static int tx(const GridJob& J) { return J.tx32; }
+    __device__ static int txf(const GridJob& J) { return J.txf32; }
     __device__ static const float2* green(const GridJob& J) { return J.green32; }
     __device__ static const float2* al(const GridJob& J) { return J.al32; }
 };
@@ -47,6 +48,7 @@ template <> struct Tile<double> {
     __device__ static int by(const GridJob& J) { return J.by64; }
     __device__ static int bz(const GridJob& J) { return J.bz64; }
     __device__ static int tx(const GridJob& J) { return J.tx64; }
+    __device__ static int txf(const GridJob& J) { return J.txf64; }
     __device__ static const double2* green(const GridJob& J) { return J.green64; }
     __device__ static const double2* al(const GridJob& J) { return J.al64; }
 };
@@ -118,24 +120,30 @@ __device__ inline void write_partials(const GridJob& J, int tile, const double (
 // Called by every thread of a one-block-per-job kernel: thread t sums tiles
 // t, t+256, ... and the block reduces, a fixed order for a given job shape.
 template <int NV>
-__device__ inline void job_sum(const GridJob& J, int ntiles, double (&out)[NV]) {
+__device__ inline void job_sum(const double* part, int stride, int ntiles, double (&out)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = 0.0;
     for (int t = threadIdx.x; t < ntiles; t += FFT_THREADS) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) out[k] += J.partial[(long)t * PARTIAL_STRIDE + k];
+        for (int k = 0; k < NV; ++k) out[k] += part[(long)t * stride + k];
     }
     block_sum<NV>(out);
+}
+template <int NV>
+__device__ inline void job_sum(const GridJob& J, int ntiles, double (&out)[NV]) {
+    job_sum<NV>(J.partial, PARTIAL_STRIDE, ntiles, out);
 }
 
 // ----------------------------------------------------------------------------
 // scalar logic (one thread of the per-job reduction), DDAModel.cpp:945-1004
 // ----------------------------------------------------------------------------
-__device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2) {
+// conv_ok: the field-error test of a refined solve passed (always true for
+// the recursive residual of an FP64 solve, which stops as the reference does)
+__device__ inline void bicg_decide(KScalars* ks, double2 rho_new, double rn2, bool conv_ok = true) {
     ks->rnorm2 = rn2;
     const double rel = sqrt(rn2 / ks->enorm2);
     const int it = ks->iter;
-    if (it >= 0 && rel <= ks->tol) {      // ||r||/||E|| <= tol after a loop step
+    if (it >= 0 && rel <= ks->tol && conv_ok) {      // ||r||/||E|| <= tol after a loop step
         ks->status = JOB_CONVERGED;
         ks->iterations = it + 1;
         ks->err = rel;
@@ -168,13 +176,18 @@ __device__ inline void step_decide(KScalars* ks, double2 rho_new, double rn2) {
     if (ks->refined) {
         const double rel2 = rn2 / ks->enorm2;
         const bool trigger = rn2 <= ks->ru_delta * ks->ru_delta * ks->maxr2 ||
-                             rel2 <= ks->tol * ks->tol || ks->iter + 1 >= ks->maxit;
+                             rel2 <= ks->res_target * ks->res_target || ks->iter + 1 >= ks->maxit;
         if (trigger && ks->iter >= 0) {
             ks->need_ru = 1;     // true residual decides (k_xinv XI_RESID)
             ks->rnorm2 = rn2;
             return;
         }
         if (rn2 > ks->maxr2) ks->maxr2 = rn2;
+        // only a reliable update (true residual and field-error estimate)
+        // ends a refined solve; the recursive residual may sit between the
+        // trigger target and tol
+        bicg_decide(ks, rho_new, rn2, false);
+        return;
     }
     bicg_decide(ks, rho_new, rn2);
 }
@@ -210,24 +223,29 @@ __device__ inline XfScalars xf_scalars(const GridJob& J) {
 template <typename KC>
 constexpr bool kSplitSolution = sizeof(KC) == sizeof(float2);
 
-// P + dP into P (dP cleared); returns the new P
+// P + dP into P (dP cleared); returns the new P and the change in *delta
 template <typename KC>
-__device__ inline double2 fold_solution(const GridJob& J, unsigned vi, double2 add) {
+__device__ inline double2 fold_solution(const GridJob& J, unsigned vi, double2 add, double2* delta = nullptr) {
     double2 Pn = cadd(gload<double2>(J.P, vi), add);
+    double2 d = add;
     if constexpr (kSplitSolution<KC>) {
-        Pn = cadd(Pn, to_d(gload<KC>(J.dP, vi)));
+        const double2 dp = to_d(gload<KC>(J.dP, vi));
+        Pn = cadd(Pn, dp);
+        d = cadd(d, dp);
         gstore<KC>(J.dP, vi, mk<KC>(0.0, 0.0));
     }
     gstore<double2>(J.P, vi, Pn);
+    if (delta) *delta = d;
     return Pn;
 }
 
 // the pending step update of one element, with dP folded into P; returns
-// the new P
+// the new P (and the change in *delta)
 template <typename KC>
-__device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfScalars& sc, double2 pv, bool write_r) {
+__device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfScalars& sc, double2 pv, bool write_r,
+                                        double2* delta = nullptr) {
     const double2 td = to_d(gload<KC>(J.t, vi));
-    const double2 Pn = fold_solution<KC>(J, vi, cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)));
+    const double2 Pn = fold_solution<KC>(J, vi, cadd(dcmul(sc.alpha, pv), dcmul(sc.zeta, td)), delta);
     if (write_r) {
         const double2 sd = to_d(gload<KC>(J.s, vi));
         gstore<KC>(J.r, vi, from_d<KC>(csub(td, dcmul(sc.zeta, sd))));
@@ -236,18 +254,29 @@ __device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfS
 }
 
 template <typename C, typename KC, int MODE>
-__device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc) {
+__device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc, double (&xacc)[XF_NACC]) {
     if (vi < 0) return mk<C>(0.0, 0.0);
     const unsigned i = (unsigned)vi;
     if (MODE == XF_PLAIN) {
         return gload<C>(J.xin, i);
     } else if (MODE == XF_SOL) {
+        double2 delta = make_double2(0.0, 0.0);
+        double2 Pn;
         if (sc.pending) {
             const double2 pv = to_d(gload<KC>(J.p, i));
-            return from_d<C>(apply_pending<KC>(J, i, sc, pv, false));
+            Pn = apply_pending<KC>(J, i, sc, pv, false, &delta);
+        } else if constexpr (kSplitSolution<KC>) {
+            Pn = fold_solution<KC>(J, i, make_double2(0.0, 0.0), &delta);
+        } else {
+            Pn = gload<double2>(J.P, i);
         }
-        if constexpr (kSplitSolution<KC>) return from_d<C>(fold_solution<KC>(J, i, make_double2(0.0, 0.0)));
-        return from_d<C>(gload<double2>(J.P, i));
+        if (J.partial_xf != nullptr) {
+            const double2 f = J.efac != nullptr ? to_d(gload<float2>(J.efac, i / 3u)) : make_double2(1.0, 0.0);
+            const double2 fd = dcmul(f, delta), fp = dcmul(f, Pn);
+            xacc[0] += fd.x * fd.x + fd.y * fd.y;
+            xacc[1] += fp.x * fp.x + fp.y * fp.y;
+        }
+        return from_d<C>(Pn);
     } else if (MODE == XF_P) {
         const double2 pv = to_d(gload<KC>(J.p, i));
         double2 rv;
@@ -356,8 +385,49 @@ __device__ inline void xi_finish(const GridJob& J, int tile, double (&acc)[XI_NA
 //   XI_RESID : true residual of the updated P (the X forward pass applied the
 //              pending update and folded dP into P): reliable-update
 //              bookkeeping and decision
+// Block sums of the X forward partials of a reliable update (all threads call it).
 template <int MODE>
-__device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int* ru_jobs) {
+__device__ inline void xf_finish(const GridJob& J, int tile, double (&xacc)[XF_NACC]) {
+    if (MODE != XF_SOL || J.partial_xf == nullptr) return;   // uniform over the block (one job)
+    block_sum<XF_NACC>(xacc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < XF_NACC; ++k) J.partial_xf[(long)tile * XF_NACC + k] = xacc[k];
+    }
+}
+
+// Field-error bookkeeping of a reliable update (refined mode): xs holds
+// ||f dP||^2 and ||f P||^2 of the update (dP = the change since the previous
+// one), rel its true relative residual.  Returns whether the field-error
+// target is met.
+__device__ inline bool error_decide(KScalars* ks, const double (&xs)[XF_NACC], double rel) {
+    if (xs[1] > 0.0) {
+        const double de = sqrt(xs[0] / xs[1]);
+        if (ks->prev_true > 0.0 && rel <= kAmpSeparation * ks->prev_true) ks->amp = de / (ks->prev_true - rel);
+        ks->prev_true = rel;
+    }
+    ks->err_est = ks->amp * rel;
+    if (ks->tol_e <= 0.0) {
+        ks->res_target = ks->tol;
+        return true;
+    }
+    ks->res_target = fmin(ks->tol, ks->tol_e / (kErrSafety * ks->amp));
+    return kErrSafety * ks->err_est <= ks->tol_e;
+}
+
+// Advance a job's scalars from the summed dot partials (one thread):
+//   XI_V     : alpha = <r0,r>/<r0,Ap>; the pending update was applied by the
+//              X forward pass of this matvec
+//   XI_S     : zeta = <At,t>/<At,At>; with r = t - zeta s,
+//              <r0, r> = <r0,t> - zeta <r0,s> and
+//              |r|^2 = |t|^2 - |<s,t>|^2/|s|^2 (zeta minimises |t - zeta s|);
+//              the vector update becomes pending and the step is decided
+//   XI_RESID : true residual of the updated P (the X forward pass applied the
+//              pending update and folded dP into P): reliable-update
+//              bookkeeping, field-error estimate and decision
+template <int MODE>
+__device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int* ru_jobs,
+                                 const double (&xs)[XF_NACC]) {
     const double2 dot = make_double2(sum[0], sum[1]);
     if (MODE == XI_V) {
         ks->alpha = ddiv(ks->rho, dot);
@@ -381,7 +451,8 @@ __device__ inline void xi_decide(KScalars* ks, const double (&sum)[XI_NACC], int
         ks->need_ru = 0;
         ks->n_ru += 1;
         ks->maxr2 = sum[2];
-        if (ks->status == JOB_ACTIVE) bicg_decide(ks, dot, sum[2]);
+        const bool err_ok = error_decide(ks, xs, sqrt(sum[2] / ks->enorm2));
+        if (ks->status == JOB_ACTIVE) bicg_decide(ks, dot, sum[2], err_ok);
     }
 }
 

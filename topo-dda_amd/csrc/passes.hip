@@ -43,14 +43,16 @@ k_xfwd(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, const cx
     const int bp = np | 1;
     const int nel = 3 * nx * nb;
     const XfScalars sc = xf_scalars<MODE>(J);
+    double xacc[XF_NACC] = {};
     for (int e = threadIdx.x; e < nel; e += FFT_THREADS) {
         const int c = e % 3;
         const int rem = e / 3;
         const int x = rem % nx;
         const int b = rem / nx;
         const int vi = vec_index(J, q0 + b, x, c);
-        lds[x * bp + 3 * b + c] = xf_element<C, KC, MODE>(J, vi, sc);
+        lds[x * bp + 3 * b + c] = xf_element<C, KC, MODE>(J, vi, sc, xacc);
     }
+    xf_finish<MODE>(J, jt.y, xacc);
     __syncthreads();
     lds_fft<-1, CAP, 1>(J.px, lds, bp, np, nx, tw);
     C* g = static_cast<C*>(J.grid);
@@ -366,7 +368,10 @@ k_xi_reduce(const GridJob* __restrict__ jobs, int gate, int* __restrict__ ru_job
     if (!gate_open(J, gate)) return;
     double sum[XI_NACC];
     job_sum<XI_NACC>(J, Tile<Real>::tx(J), sum);
-    if (threadIdx.x == 0) xi_decide<MODE>(J.ks, sum, ru_jobs);
+    // a reliable update's X forward sums (its tiles are the X forward tiles)
+    double xs[XF_NACC] = {0.0, 0.0};
+    if (MODE == XI_RESID && J.partial_xf != nullptr) job_sum<XF_NACC>(J.partial_xf, XF_NACC, Tile<Real>::txf(J), xs);
+    if (threadIdx.x == 0) xi_decide<MODE>(J.ks, sum, ru_jobs, xs);
 }
 
 // ----------------------------------------------------------------------------
@@ -427,6 +432,10 @@ k_init_reduce(const GridJob* __restrict__ jobs, int warm) {
     ks->n_ru = 0;
     ks->iterations = 0;
     ks->err = 0.0;
+    ks->amp = kAmpPrior;
+    ks->prev_true = 0.0;   // the first reliable update only sets the reference level
+    ks->err_est = 0.0;
+    ks->res_target = ks->tol_e > 0.0 ? fmin(ks->tol, ks->tol_e / (kErrSafety * kAmpPrior)) : ks->tol;
     // a zero right-hand side has the exact solution P = 0
     ks->status = (sum[1] == 0.0 && !warm) ? JOB_CONVERGED : JOB_ACTIVE;
 }

@@ -38,11 +38,13 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
         const int q0 = jt.y * fast_nr_x<Real>(L1, L2);
         const int nr = min(fast_nr_x<Real>(L1, L2), J.ny * J.nz - q0);
         const XfScalars sc = xf_scalars<MODE>(J);
+        double xacc[XF_NACC] = {};
         __syncthreads();
         for (int t = threadIdx.x; t < X::TA; t += FFT_THREADS)
             X::fwd_a(t, lds, twl, nr, J.nx, [&](int r, int c, int x) {
-                return xf_element<C, KC, MODE>(J, vec_index(J, q0 + r, x, c), sc);
+                return xf_element<C, KC, MODE>(J, vec_index(J, q0 + r, x, c), sc, xacc);
             });
+        xf_finish<MODE>(J, jt.y, xacc);
         __syncthreads();
         C* g = static_cast<C*>(J.grid);
         const unsigned ny = J.ny, ly = J.ly;

@@ -39,12 +39,17 @@ __device__ inline double2 inv_alpha(const SetupParams& P, double2 eps) {
 }
 
 __global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, double2* __restrict__ al64,
-                                float2* __restrict__ al32, double2* __restrict__ rhs) {
+                                float2* __restrict__ al32, double2* __restrict__ rhs, float2* __restrict__ efac) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
-    const double2 al = inv_alpha(P, eps_of(P, rho[i]));
+    const double2 eps = eps_of(P, rho[i]);
+    const double2 al = inv_alpha(P, eps);
     al64[i] = al;
     al32[i] = make_float2((float)al.x, (float)al.y);
+    if (efac != nullptr) {   // E = f P, f = al 3/(eps + 2) (DDAModel.cpp:1185-1192)
+        const double2 f = ddiv(al, make_double2((eps.x + 2.0) / 3.0, eps.y / 3.0));
+        efac[i] = make_float2((float)f.x, (float)f.y);
+    }
     const int gx = i % P.nx, gy = (i / P.nx) % P.ny, gz = i / (P.nx * P.ny);
     const double ph = P.K * P.d * (P.nk[0] * gx + P.nk[1] * gy + P.nk[2] * gz);
     double s, c;
@@ -133,8 +138,9 @@ void Problem::set_density(const double* r, bool on_device, hipStream_t st) {
         rho.upload(r, ndip, st);
     SetupParams P = params_of(*this);
     hipLaunchKernelGGL(k_problem_setup, dim3((ndip + 255) / 256), dim3(256), 0, st, P, rho.get(), sys->al64.get(),
-                       sys->al32.get(), sys->rhs.get());
+                       sys->al32.get(), sys->rhs.get(), sys->efac.get());
     TDDA_HIP(hipGetLastError());
+    sys->job.efac = sys->efac.get();
 }
 
 void Problem::field(double2* out, hipStream_t st) const {

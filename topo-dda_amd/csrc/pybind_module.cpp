@@ -181,6 +181,7 @@ class PyDDAModel {
         d["error"] = s.error;
         d["reliable_updates"] = s.reliable_updates;
         d["ms"] = s.ms;
+        d["err_est"] = s.err_est;
         return d;
     }
     py::array_t<cd> getPolarization() {
@@ -220,6 +221,7 @@ class PyDDAModel {
         d["error"] = s.error;
         d["reliable_updates"] = s.reliable_updates;
         d["ms"] = s.ms;
+        d["err_est"] = s.err_est;
         return d;
     }
     std::vector<double> timePasses(int reps) {

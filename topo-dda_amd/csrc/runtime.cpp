@@ -288,39 +288,60 @@ void OperatorCache::clear() {
     lru_.clear();
 }
 
+void OperatorCache::touch_locked(const OperatorKey& key) {
+    auto pos = std::find_if(lru_.begin(), lru_.end(),
+                            [&](const OperatorKey& k) { return !(k < key) && !(key < k); });
+    if (pos != lru_.end()) lru_.erase(pos);
+    lru_.push_back(key);
+}
+
+// One thread builds a missing operator; others asking for the same key wait
+// for it instead of building a second copy.
 std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st,
                                              bool* hit) {
     {
-        std::lock_guard<std::mutex> lk(mu_);
+        std::unique_lock<std::mutex> lk(mu_);
+        built_.wait(lk, [&] { return building_.count(key) == 0; });
         auto it = map_.find(key);
         if (it != map_.end()) {
-            auto pos = std::find_if(lru_.begin(), lru_.end(), [&](const OperatorKey& k) {
-                return !(k < key) && !(key < k);
-            });
-            if (pos != lru_.end()) lru_.erase(pos);
-            lru_.push_back(key);
+            touch_locked(key);
             if (hit) *hit = true;
             return it->second;
         }
+        building_.insert(key);
     }
     if (hit) *hit = false;
-    auto op = std::make_shared<Operator>(key, tables, st);
-    std::lock_guard<std::mutex> lk(mu_);
-    map_[key] = op;
-    lru_.push_back(key);
-    size_t total = 0;
-    for (auto& kv : map_) total += kv.second->bytes();
-    // evict least recently used entries nobody else holds
-    for (size_t i = 0; i < lru_.size() && total > budget_;) {
-        auto it = map_.find(lru_[i]);
-        if (it != map_.end() && it->second.use_count() == 1 && it->second != op) {
-            total -= it->second->bytes();
-            map_.erase(it);
-            lru_.erase(lru_.begin() + i);
-        } else {
-            ++i;
+    std::shared_ptr<Operator> op;
+    try {
+        op = std::make_shared<Operator>(key, tables, st);
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            building_.erase(key);
+        }
+        built_.notify_all();
+        throw;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        building_.erase(key);
+        map_[key] = op;
+        touch_locked(key);
+        size_t total = 0;
+        for (auto& kv : map_) total += kv.second->bytes();
+        // evict least recently used entries nobody else holds
+        for (size_t i = 0; i < lru_.size() && total > budget_;) {
+            auto it = map_.find(lru_[i]);
+            if (it != map_.end() && it->second.use_count() == 1 && it->second != op) {
+                total -= it->second->bytes();
+                map_.erase(it);
+                lru_.erase(lru_.begin() + i);
+            } else {
+                ++i;
+            }
         }
     }
+    built_.notify_all();
     return op;
 }
 
@@ -337,6 +358,7 @@ size_t System::estimate_bytes(const Operator& op, int ndip, bool refined) {
     b += 3 * n3 * sizeof(double2);   // P, rhs, scratch64
     b += ndip * (sizeof(double2) + sizeof(float2));
     b += (size_t)std::max<long>(n3 / (FFT_THREADS * UPD_EPT) + 1, (long)op.ny * op.nz) * 6 * sizeof(double);
+    if (refined) b += (size_t)op.ny * op.nz * XF_NACC * sizeof(double) + ndip * sizeof(float2);
     return b;
 }
 
@@ -362,7 +384,11 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.px = o.px; job.py = o.py; job.pz = o.pz;
     plan_tiles(job);
     const long nparts = std::max({(long)job.tupd, (long)job.tx32, (long)job.tx64});
-    partial.alloc(nparts * PARTIAL_STRIDE);
+    // reliable updates run the FP64 X forward pass: its tiles hold the
+    // field-error partials
+    const long nxf = refined ? (long)job.txf64 * XF_NACC : 0;
+    partial.alloc(nparts * PARTIAL_STRIDE + nxf);
+    if (refined) efac.alloc(ndip);
     if (box_map_host) {
         box_map.alloc((size_t)o.nx * o.ny * o.nz);
         box_map.upload(box_map_host, box_map.size(), st);
@@ -388,12 +414,14 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.P = P.get();
     job.rhs = rhs.get();
     job.partial = partial.get();
+    job.partial_xf = refined ? partial.get() + nparts * PARTIAL_STRIDE : nullptr;
+    job.efac = nullptr;   // f = 1 until set_field_factor
     job.ks = nullptr;
 }
 
 size_t System::bytes() const {
     return grid.bytes() + kvec.bytes() + P.bytes() + rhs.bytes() + scratch64.bytes() + al64.bytes() +
-           al32.bytes() + partial.bytes() + box_map.bytes();
+           al32.bytes() + partial.bytes() + efac.bytes() + box_map.bytes();
 }
 
 namespace {
@@ -413,6 +441,15 @@ void System::set_alpha_device(const double2* al_dip_dev, hipStream_t st) {
     TDDA_HIP(hipMemcpyAsync(al64.get(), al_dip_dev, ndip * sizeof(double2), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_alpha32, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(), (long)ndip);
     TDDA_HIP(hipGetLastError());
+}
+
+void System::set_field_factor(const double2* f, hipStream_t st) {
+    if (!refined) return;
+    std::vector<float2> h(ndip);
+    for (int i = 0; i < ndip; ++i) h[i] = make_float2((float)f[i].x, (float)f[i].y);
+    efac.upload(h.data(), ndip, st);
+    TDDA_HIP(hipStreamSynchronize(st));   // h leaves scope
+    job.efac = efac.get();
 }
 
 void System::set_rhs(const double2* h, hipStream_t st) { rhs.upload(h, 3ull * ndip, st); }
@@ -648,6 +685,7 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
         hks[j].maxit = opt.maxit;
         hks[j].refined = opt.refined ? 1 : 0;
         hks[j].ru_delta = opt.ru_delta;
+        hks[j].tol_e = opt.refined ? opt.err_tol : 0.0;
         hks[j].status = JOB_ACTIVE;
     }
     ks.upload(hks.data(), nj, st);
@@ -697,6 +735,7 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
         res[j].err = snap[j].err;
         res[j].n_ru = snap[j].n_ru;
         res[j].ms = ms;
+        res[j].err_est = snap[j].err_est;
     }
     return res;
 }
@@ -724,6 +763,57 @@ void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st) {
     b.upload_jobs(st);
     launch_matvec(b.ls, PlanRegistry::instance().pools(), true, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
     TDDA_HIP(hipStreamSynchronize(st));
+}
+
+void apply_batch(const std::vector<System*>& v, bool c64, const double2* x, double2* y, hipStream_t st) {
+    TDDA_REQUIRE(!v.empty(), ERR_INVALID, "no systems");
+    const bool refined = v[0]->refined;
+    for (System* s : v) {
+        TDDA_REQUIRE(s->refined == refined, ERR_INVALID, "mixed precisions in one batch");
+        TDDA_REQUIRE(!c64 || s->refined, ERR_INVALID, "c64 transforms need refined systems");
+    }
+    Batch b;
+    b.build(v, nullptr, refined, st);
+    const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
+    std::vector<DevBuf<char>> xs(v.size()), ys(v.size());
+    std::vector<std::vector<float2>> h32(v.size());
+    size_t off = 0;
+    for (size_t j = 0; j < v.size(); ++j) {
+        const size_t n3 = 3ull * v[j]->ndip;
+        xs[j].alloc(n3 * cs);
+        ys[j].alloc(n3 * cs);
+        if (c64) {
+            h32[j].resize(n3);
+            for (size_t i = 0; i < n3; ++i) h32[j][i] = make_float2((float)x[off + i].x, (float)x[off + i].y);
+            TDDA_HIP(hipMemcpyAsync(xs[j].get(), h32[j].data(), n3 * cs, hipMemcpyHostToDevice, st));
+        } else {
+            TDDA_HIP(hipMemcpyAsync(xs[j].get(), x + off, n3 * cs, hipMemcpyHostToDevice, st));
+        }
+        b.hjobs[j].xin = xs[j].get();
+        b.hjobs[j].yout = ys[j].get();
+        off += n3;
+    }
+    b.upload_jobs(st);
+    LaunchSet l = b.ls;
+    l.refined = refined;
+    launch_matvec(l, PlanRegistry::instance().pools(), c64, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
+    off = 0;
+    for (size_t j = 0; j < v.size(); ++j) {
+        const size_t n3 = 3ull * v[j]->ndip;
+        if (c64)
+            TDDA_HIP(hipMemcpyAsync(h32[j].data(), ys[j].get(), n3 * cs, hipMemcpyDeviceToHost, st));
+        else
+            TDDA_HIP(hipMemcpyAsync(y + off, ys[j].get(), n3 * cs, hipMemcpyDeviceToHost, st));
+        off += n3;
+    }
+    TDDA_HIP(hipStreamSynchronize(st));
+    if (c64) {
+        off = 0;
+        for (size_t j = 0; j < v.size(); ++j) {
+            for (size_t i = 0; i < h32[j].size(); ++i) y[off + i] = make_double2(h32[j][i].x, h32[j][i].y);
+            off += h32[j].size();
+        }
+    }
 }
 
 void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5) {

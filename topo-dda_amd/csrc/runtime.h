@@ -2,9 +2,11 @@
 // and the batched BiCGSTAB driver.
 #pragma once
 
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <vector>
 
 #include "dda_types.h"
@@ -137,9 +139,12 @@ class OperatorCache {
     size_t trim(size_t bytes);
 
    private:
+    void touch_locked(const OperatorKey& key);   // move key to the most recent end of lru_
     mutable std::mutex mu_;
+    std::condition_variable built_;
     std::map<OperatorKey, std::shared_ptr<Operator>> map_;
-    std::vector<OperatorKey> lru_;
+    std::set<OperatorKey> building_;   // keys one thread is building (others wait for it)
+    std::vector<OperatorKey> lru_;     // each cached key once, least recent first
     size_t budget_ = size_t(8) << 30;
 };
 
@@ -153,6 +158,9 @@ class System {
     System(std::shared_ptr<Operator> op, int ndip, const int* box_map_host, bool refined, hipStream_t st);
     void set_alpha(const double2* al_dip_host, hipStream_t st);   // 1/alpha per dipole
     void set_alpha_device(const double2* al_dip_dev, hipStream_t st);
+    // local-field factor f = al 3/(eps + 2) per dipole (refined systems): the
+    // field-error estimate weighs the solution by it (E = f P)
+    void set_field_factor(const double2* f_dip_host, hipStream_t st);
     void set_rhs(const double2* rhs_host, hipStream_t st);
     void set_solution(const double2* P_host, hipStream_t st);
     void get_solution(double2* P_host, hipStream_t st) const;
@@ -168,7 +176,8 @@ class System {
     DevBuf<double2> P, rhs, scratch64;
     DevBuf<double2> al64;
     DevBuf<float2> al32;
-    DevBuf<double> partial;
+    DevBuf<double> partial;     // X inverse partials, then (refined) the X forward ones
+    DevBuf<float2> efac;        // refined: local-field factor per dipole
     DevBuf<int> box_map;
 };
 
@@ -177,6 +186,9 @@ struct SolveOptions {
     int maxit = 100000;
     double tol = 1e-5;
     double ru_delta = 0.1;
+    // refined mode: relative field-error target (estimated from the solve
+    // itself, see KScalars); 0: stop on the true residual alone
+    double err_tol = 0.0;
     bool warm = false;      // use the system's current P as the initial guess
     int chunk = 0;          // iterations between host checks (0: adaptive)
 };
@@ -187,6 +199,7 @@ struct SolveResult {
     double err = 0.0;
     int n_ru = 0;
     double ms = 0.0;
+    double err_est = 0.0;   // refined mode: estimated relative field error (0: none)
 };
 
 // Device-resident batch: descriptors and (job, tile) maps for every pass.
@@ -218,6 +231,10 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
 void apply_operator64(System& s, const double2* x, double2* y, hipStream_t st);
 // same with c64 transforms (requires a refined system)
 void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st);
+// y_j = A_j x_j + al_j x_j for every system in one batched launch per pass;
+// x, y: host vectors of the systems back to back (3 ndip_j complex each);
+// c64 transforms need refined systems
+void apply_batch(const std::vector<System*>& v, bool c64, const double2* x, double2* y, hipStream_t st);
 
 // Average ms per launch of the five passes (X fwd, Y fwd, Z, Y inv, X inv)
 // of y = A x + al x over `reps` repetitions, c64 or c128 transforms.

@@ -73,13 +73,14 @@ int Server::poll(std::vector<ServeResult>& out, int max) {
     return k;
 }
 
-bool Server::take_field(long id, std::vector<double2>& E) {
+int Server::take_field(long id, size_t n, std::vector<double2>& E) {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = fields_.find(id);
-    if (it == fields_.end()) return false;
+    if (it == fields_.end()) return 1;
+    if (it->second.size() != n) return 2;
     E = std::move(it->second);
     fields_.erase(it);
-    return true;
+    return 0;
 }
 
 bool Server::idle() {
@@ -169,6 +170,7 @@ void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
         r.iterations = ks.iterations;
         r.n_ru = ks.n_ru;
         r.err = ks.err;
+        r.err_est = ks.err_est;
         r.t_submit = sl.req.t_submit;
         r.t_admit = sl.t_admit;
         r.t_done = t;
@@ -247,6 +249,7 @@ void Server::loop() {
                     in.maxit = opt_.maxit;
                     in.refined = opt_.refined ? 1 : 0;
                     in.ru_delta = opt_.ru_delta;
+                    in.tol_e = opt_.refined ? opt_.err_tol : 0.0;
                     in.status = JOB_ACTIVE;
                     TDDA_HIP(hipMemcpyAsync(ks_.get() + joined[k], &in, sizeof(KScalars), hipMemcpyHostToDevice,
                                             main_));

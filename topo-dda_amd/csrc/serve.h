@@ -26,6 +26,7 @@ struct ServerOptions {
     int maxit = 100000;
     double tol = 1e-5;
     double ru_delta = 0.1;
+    double err_tol = 1e-5;           // refined: field-error target (0: true residual only)
     size_t hbm_budget = size_t(200) << 30;
     int max_slots = 64;
     int chunk = 4;                   // iterations between host checks
@@ -45,6 +46,7 @@ struct ServeResult {
     int iterations = 0;
     int n_ru = 0;
     double err = 0.0;
+    double err_est = 0.0;
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;   // seconds since server start
     int op_hit = 0;
 };
@@ -61,7 +63,8 @@ class Server {
     ~Server();
     void submit(ServeRequest&& r);
     int poll(std::vector<ServeResult>& out, int max);
-    bool take_field(long id, std::vector<double2>& E);
+    // 0: moved out; 1: no field kept for id; 2: the field has another length (kept)
+    int take_field(long id, size_t n, std::vector<double2>& E);
     bool idle();
     double now() const;
     ServerStats stats();

@@ -28,19 +28,20 @@ class ProblemDesc(ctypes.Structure):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("error", ctypes.c_double),
-                ("reliable_updates", ctypes.c_int), ("ms", ctypes.c_double)]
+                ("reliable_updates", ctypes.c_int), ("ms", ctypes.c_double), ("err_est", ctypes.c_double)]
 
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("maxit", ctypes.c_int), ("tol", ctypes.c_double),
                 ("ru_delta", ctypes.c_double), ("hbm_budget_gb", ctypes.c_double), ("max_slots", ctypes.c_int),
-                ("chunk", ctypes.c_int), ("keep_fields", ctypes.c_int)]
+                ("chunk", ctypes.c_int), ("keep_fields", ctypes.c_int), ("err_tol", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_long), ("status", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("reliable_updates", ctypes.c_int), ("error", ctypes.c_double), ("t_submit", ctypes.c_double),
-                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("operator_cached", ctypes.c_int)]
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("operator_cached", ctypes.c_int),
+                ("err_est", ctypes.c_double)]
 
 
 class ServerStats(ctypes.Structure):
@@ -71,6 +72,7 @@ def lib():
                                             ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(SolveInfo)]
         L.tdda_problems_time_passes.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_problems_time_iteration.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_problems_apply.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, dp, dp]
         L.tdda_server_create.argtypes = [ctypes.POINTER(ServerOpts), dp, dp, ctypes.c_long, ctypes.c_double,
                                          ctypes.POINTER(vp)]
         L.tdda_server_destroy.argtypes = [vp]
@@ -199,7 +201,8 @@ class Problem:
             ptr, dev = _dp(out), 0
         check(lib().tdda_problem_gradient(self.h, OBJ[kind], power, _dp(t) if t is not None else None, nt, maxit,
                                           tol, ptr, dev, ctypes.byref(val), ctypes.byref(info)))
-        return out, val.value, dict(status=STATUS[info.status], iterations=info.iterations, error=info.error)
+        return out, val.value, dict(status=STATUS[info.status], iterations=info.iterations, error=info.error,
+                                    err_est=info.err_est)
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
@@ -213,7 +216,22 @@ def solve(problems, maxit=100000, tol=1e-5):
     infos = (SolveInfo * len(problems))()
     check(lib().tdda_problems_solve(arr, len(problems), maxit, tol, infos))
     return [dict(status=STATUS[i.status], iterations=i.iterations, error=i.error,
-                 reliable_updates=i.reliable_updates, ms=i.ms) for i in infos]
+                 reliable_updates=i.reliable_updates, ms=i.ms, err_est=i.err_est) for i in infos]
+
+
+def apply(problems, xs, c64=False):
+    """y = A x + al x for every problem in one batched launch per pass, with
+    c64 transforms (refined problems) or FP64 ones; xs: one complex vector of
+    3 N entries per problem.  Returns the list of y."""
+    arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
+    x = np.ascontiguousarray(np.concatenate([np.asarray(v, np.complex128).reshape(-1) for v in xs]))
+    y = np.empty_like(x)
+    check(lib().tdda_problems_apply(arr, len(problems), int(c64), _dp(x.view(np.float64)), _dp(y.view(np.float64))))
+    out, k = [], 0
+    for p in problems:
+        out.append(y[k:k + 3 * p.n].copy())
+        k += 3 * p.n
+    return out
 
 
 def time_passes(problems, c64=True, reps=20):
@@ -237,10 +255,13 @@ class Server:
     """Continuous-batching request server (C++ loop thread)."""
 
     def __init__(self, precision="fp32_refined", maxit=100000, tol=1e-5, ru_delta=0.1, hbm_budget_gb=200.0,
-                 max_slots=64, chunk=4, keep_fields=True, tables=None):
+                 max_slots=64, chunk=4, keep_fields=True, tables=None, err_tol=0.0):
+        """err_tol: relative field-error target of refined solves (0: tol;
+        negative: stop on the true residual alone)."""
         si, ci, n, delta = _tables(tables)
         self._keep = (si, ci)
-        o = ServerOpts(PREC[precision], maxit, tol, ru_delta, hbm_budget_gb, max_slots, chunk, int(keep_fields))
+        o = ServerOpts(PREC[precision], maxit, tol, ru_delta, hbm_budget_gb, max_slots, chunk, int(keep_fields),
+                       err_tol)
         h = ctypes.c_void_p()
         check(lib().tdda_server_create(ctypes.byref(o), _dp(si) if si is not None else None,
                                        _dp(ci) if ci is not None else None, n, delta, ctypes.byref(h)))
@@ -258,7 +279,8 @@ class Server:
         check(lib().tdda_server_poll(self.h, buf, max_results, ctypes.byref(n)))
         return [dict(id=r.id, status=STATUS[r.status], iterations=r.iterations,
                      reliable_updates=r.reliable_updates, error=r.error, t_submit=r.t_submit,
-                     t_admit=r.t_admit, t_done=r.t_done, operator_cached=bool(r.operator_cached))
+                     t_admit=r.t_admit, t_done=r.t_done, operator_cached=bool(r.operator_cached),
+                     err_est=r.err_est)
                 for r in buf[:n.value]]
 
     def field(self, rid):
