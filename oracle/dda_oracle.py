@@ -627,6 +627,39 @@ class OracleFullBox:
             out.append(e - s)
         return np.array(out)
 
+    def target_fields_ldr(self, targets):
+        """target_fields for the LDR interaction, vectorised over the dipoles
+        (numpy): the same undamped single-image tensor as green_point
+        (AProductCore.cpp:351-386), E(t) = E_inc(t) - sum_j A(t - r_j) P_j.
+        tests/ check it against the loop version."""
+        if self.method != METHOD_LDR:
+            raise ValueError("target_fields_ldr evaluates the LDR interaction only")
+        g = self.geometry.reshape(-1, 3).astype(np.float64) * self.d
+        P = self.P.reshape(-1, 3)
+        K = self.K
+        out = []
+        for t in np.asarray(targets, float).reshape(-1, 3):
+            R = t[None, :] - g
+            xx, yy, zz = R[:, 0] ** 2, R[:, 1] ** 2, R[:, 2] ** 2
+            r2 = xx + yy + zz
+            r = np.sqrt(r2)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                pre = np.where(r > 0, np.exp(1j * K * r) / (r2 * r), 0.0)
+                c2 = np.where(r > 0, (1.0 - 1j * K * r) / r2, 0.0)
+            c3 = K * K - 3.0 * c2
+            a00 = pre * (c2 * (r2 - 3 * xx) - K * K * (yy + zz))
+            a11 = pre * (c2 * (r2 - 3 * yy) - K * K * (zz + xx))
+            a22 = pre * (c2 * (r2 - 3 * zz) - K * K * (xx + yy))
+            a01 = pre * c3 * R[:, 0] * R[:, 1]
+            a02 = pre * c3 * R[:, 0] * R[:, 2]
+            a12 = pre * c3 * R[:, 1] * R[:, 2]
+            s = np.array([np.sum(a00 * P[:, 0] + a01 * P[:, 1] + a02 * P[:, 2]),
+                          np.sum(a01 * P[:, 0] + a11 * P[:, 1] + a12 * P[:, 2]),
+                          np.sum(a02 * P[:, 0] + a12 * P[:, 1] + a22 * P[:, 2])])
+            ph = K * float(self.nK @ t)
+            out.append(self.E0 * self.nE0 * complex(math.cos(ph), math.sin(ph)) - s)
+        return np.array(out)
+
     def objective_targets(self, targets):
         E = self.target_fields(targets)
         return float(np.sum(np.abs(E) ** 2)) / E.shape[0]

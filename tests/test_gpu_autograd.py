@@ -129,3 +129,35 @@ def test_refined_gradient_close_to_fp64(native):
     err = np.linalg.norm(grads["fp32_refined"] - grads["fp64"]) / np.linalg.norm(grads["fp64"])
     _log(f"[gpu] refined gradient rel err {err:.3e}")
     assert err < 1e-5
+
+
+def test_gradient_outside_unit_interval(native):
+    """Densities are clipped to [0, 1] before the permittivity mix, so the
+    objective is flat in a voxel whose density lies outside: the gradient
+    there is zero, and central differences agree at such voxels and inside."""
+    _log("[gpu] gradient outside [0, 1]")
+    shape = (8, 6, 4)
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.1, 0.9, int(np.prod(shape)))
+    out_idx = {3: -0.1, 40: 1.2, 77: 1.0 + 1e-3}
+    for i, v in out_idx.items():
+        rho[i] = v
+    req = native.Request(shape=shape, d=12.0, lam=542.0, rho=rho)
+    p = native.Problem(req, precision="fp64")
+    p.solve(tol=1e-13)
+    g, val, _ = p.gradient(kind="integrated", tol=1e-13)
+    h = 1e-4
+    for i in list(out_idx) + [10, 55]:
+        vals = []
+        for s in (1, -1):
+            r = rho.copy()
+            r[i] += s * h
+            p.set_density(r)
+            p.solve(tol=1e-13)
+            vals.append(p.objective(kind="integrated"))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        if i in out_idx:
+            assert g[i] == 0.0 and abs(fd) < 1e-9, (i, g[i], fd)
+        else:
+            assert g[i] == pytest.approx(fd, rel=1e-5), (i, g[i], fd)
+    p.set_density(rho)

@@ -230,6 +230,85 @@ def test_bicgstab_solves_dense_system(tables):
     assert info2.error <= 1e-3 and info2.iterations <= info.iterations
 
 
+def _textbook_bicgstab(A, b, steps):
+    """BiCGSTAB as published (van der Vorst 1992, Algorithm 3; inner product
+    (u, w) = sum conj(u) w), written out independently of the oracle: returns
+    the iterate and the relative residual after each of `steps` iterations."""
+    x = np.zeros_like(b)
+    r = b - A @ x
+    rhat = r.copy()
+    rho_old = alpha = omega = 1.0 + 0j
+    v = np.zeros_like(b)
+    p = np.zeros_like(b)
+    xs, res = [], []
+    for _ in range(steps):
+        rho = np.vdot(rhat, r)
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        v = A @ p
+        alpha = rho / np.vdot(rhat, v)
+        s = r - alpha * v
+        t = A @ s
+        omega = np.vdot(t, s) / np.vdot(t, t)
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        rho_old = rho
+        xs.append(x.copy())
+        res.append(np.linalg.norm(r) / np.linalg.norm(b))
+    return xs, res
+
+
+def test_bicgstab_recurrence_is_textbook(tables):
+    """The reference's BiCGSTAB (DDAModel.cpp:899-1009: a first step before
+    the loop with p = r, then beta = (alpha/zeta) <r0,r>/<r0,rl>,
+    p = r + beta (p - zeta v), P += alpha p + zeta r - alpha zeta v) is the
+    textbook recurrence: same iterates and residuals step by step on a small
+    dense DDA system (a dropped term, a swapped alpha/zeta or a conjugation
+    on the wrong side changes them at O(1)).  The system is well conditioned
+    (cond ~ 20) so that rounding, which the recurrence amplifies, stays below
+    1e-9 over the ten steps compared."""
+    K = 2 * math.pi / 800.0
+    nx, ny, nz = 3, 3, 2
+    cent = O.green_offsets(nx, ny, nz, 15.0, K, O.METHOD_LDR, 0, 0, 0.0, 0.0)
+    geo = O.generate_geometry(nx, ny, nz)
+    n = 3 * nx * ny * nz
+    rng = np.random.default_rng(2)
+    al = 1.0 / O.get_alpha(K, 15.0, 1.0 + rng.uniform(0, 1, n) * 0.4 * (4.96 + 3.89j), [1, 0, 0], [0, 0, 1])
+    A = O.dense_interaction_matrix(cent, geo) + np.diag(al)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    steps = 10
+    xs, res = _textbook_bicgstab(A, b, steps)
+    for k in range(steps):
+        # maxit = k loop steps after the first step = textbook iteration k + 1
+        P, _ = O.bicgstab_reference(lambda v: A @ v, b, np.zeros(n, complex), k, 0.0)
+        assert np.linalg.norm(P - xs[k]) <= 1e-8 * np.linalg.norm(xs[k]), k
+    hist = []
+    O.bicgstab_reference(lambda v: A @ v, b, np.zeros(n, complex), steps - 1, 0.0, history=hist)
+    assert np.allclose(hist, res[1:steps], rtol=1e-8)
+    assert res[-1] < 1e-4 * res[0]
+
+
+def test_target_fields_local_field_identity():
+    """At a dipole site the field of all the other dipoles plus the incident
+    field is the local field that polarises it: E_inc,i - sum_{j != i}
+    A(r_i - r_j) P_j = al_i P_i for a solved system (DDAModel.cpp:1153-1192
+    evaluated at r_i; the interaction is zero at r = 0).  The solve's tensor
+    carries the damping exp(-(0.01 K r)^4), which is 1 - 1e-8 over this box,
+    and target_fields does not, hence the tolerance.  The vectorised LDR
+    evaluation equals the loop version."""
+    rng = np.random.default_rng(3)
+    shape = (5, 4, 3)
+    box = O.OracleFullBox(shape, 12.0, 542.0, rng.uniform(0, 1, 60), n_k=(0.0, 0.6, 0.8), n_e0=(1.0, 0.0, 0.0))
+    box.solve(tol=1e-14)
+    sites = box.geometry.reshape(-1, 3)[[0, 17, 33, 59]] * 12.0
+    E = box.target_fields(sites)
+    want = (box.al * box.P).reshape(-1, 3)[[0, 17, 33, 59]]
+    assert np.allclose(E, want, rtol=1e-6, atol=1e-9 * np.abs(want).max())
+    tg = np.array([[20.0, 18.0, 60.0], [0.0, 0.0, -40.0], [13.0, 7.0, 5.0]])
+    assert np.allclose(box.target_fields_ldr(tg), box.target_fields(tg), rtol=1e-12, atol=1e-15)
+    assert np.allclose(box.target_fields_ldr(sites), E, rtol=1e-12, atol=1e-15)
+
+
 def test_bicgstab_maxit_zero_and_zero_rhs():
     A = np.eye(3) * (2 + 1j)
     b = np.ones(3, complex)
