@@ -8,12 +8,18 @@ uniform in [0, 1) per voxel, LDR interaction, non-periodic.  Every request is
 solved to a 1e-5 relative residual (true FP64 residual) by the continuous-
 batching server in complex64 with FP64 reliable updates.
 
-A step is one pass over the R requests with all of them queued at once (the
+A step is one pass over R requests of its own (fresh seeds every step, so
+every request pays its Green's set-up) with all of them queued at once (the
 server's saturation throughput); `value` is solves/s over the K timed steps.
-The same requests then arrive as a Poisson stream at 0.8x that rate, which
-gives the p50/p99 latency.  The plain FP64 single-solve GPU path (one request
-at a time, FP64 transforms and Krylov vectors) is timed on a subset of the
-same requests for the speedup, and the FP64 CPU oracle on a bounded sample.
+Another R requests then arrive as a Poisson stream at 0.8x that rate, which
+gives the p50/p99 latency.  Each refined solve stops when its FP64 true
+residual meets 1e-5 and its estimated field error meets 1e-5; the fields of
+a subset are compared with a tight FP64 solve.  The plain FP64 single-solve
+GPU path (one request at a time, FP64 transforms and Krylov vectors, the
+reference's stopping rule) is timed on the same subset for the speedup, at
+tol 1e-5 and at the tolerance that gives the same field accuracy.  Beside
+them: the 128x128x64 eight-RHS solve, the FP64 lattice-sum rate, a device
+copy, and the FP64 CPU oracle on a bounded sample (1 and 16 threads).
 """
 import argparse
 import json
@@ -72,10 +78,13 @@ def percentile(v, q):
     return s[lo] + (s[hi] - s[lo]) * (k - lo)
 
 
-def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=False, timeout=600.0):
+def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=False, timeout=600.0, tol=1e-5,
+               err_tol=0.0, precond="none"):
     """Submit every request (at its arrival time if given) and wait for all.
-    Returns (wall seconds from first submission to last completion, results)."""
-    srv = native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields)
+    Returns (wall seconds from first submission to last completion, results,
+    server statistics, fields or None)."""
+    srv = native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields, tol=tol,
+                        err_tol=err_tol, precond=precond)
     results = {}
     t0 = time.perf_counter()
     nxt = 0
@@ -97,8 +106,27 @@ def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=
     return wall, results, stats, out_fields
 
 
-def cpu_baseline(reqs, iterations, per_shape_s=4.0):
-    """FP64 oracle (numpy + scipy.fft, one worker) timed on a bounded sample.
+def host_cpu():
+    """nproc and the lscpu model name of this host."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "model": None}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    return info
+
+
+def cpu_baseline(reqs, iterations, workers=1, per_shape_s=4.0):
+    """FP64 oracle (numpy + scipy.fft with `workers` FFT threads) timed on a
+    bounded sample.
 
     For the first request of every box shape in the stream, the oracle's set-up
     (Green's tensor + its FFT) and its BiCGSTAB iterations are timed, running
@@ -115,7 +143,7 @@ def cpu_baseline(reqs, iterations, per_shape_s=4.0):
                            r.lam, symmetry_method="None", apc_max_m=0, apc_max_n=0, apc_method_name="LDR",
                            tables=(np.zeros(2), np.zeros(2), 0.1))
         t0 = time.perf_counter()
-        m = O.OracleDDAModel(*a, workers=1)
+        m = O.OracleDDAModel(*a, workers=workers)
         setup = time.perf_counter() - t0
         k, t_it = 1, 0.0
         while True:
@@ -134,6 +162,36 @@ def cpu_baseline(reqs, iterations, per_shape_s=4.0):
     desc = ", ".join(f"{'x'.join(map(str, s))}: setup {v[0]:.2f} s, {v[1] * 1e3:.1f} ms/iteration ({v[2]} matvecs)"
                      for s, v in per_shape.items())
     return len(reqs) / total, sample_s, desc
+
+
+def lattice_sum_rates(native, tables, threads):
+    """FP64 Green's-tensor lattice sum of the periodic unit cell (32x32x12,
+    d = 10 nm, 600 nm, FCD, period 32 cells), in Green evaluations per second
+    = (2Nx-1)(2Ny-1)(2Nz-1)(2 MAXm+1)(2 MAXn+1) / time.  CPU: the oracle's C
+    sum (OpenMP, `threads` threads, octant mode) at +-8 images, a bounded
+    sample; GPU: the device operator set-up at the full +-50 images (lattice
+    sum, DFTs and spectrum layout, timed by the library)."""
+    import dda_oracle as O
+    shape, d, lam = (32, 32, 12), 10.0, 600.0
+    m_ref = (2 * shape[0] - 1) * (2 * shape[1] - 1) * (2 * shape[2] - 1)
+    si, ci, delta = tables
+    K = 2 * math.pi / lam
+    cpu_images = 8
+    t0 = time.perf_counter()
+    O.green_offsets(*shape, d, K, O.METHOD_FCD, cpu_images, cpu_images, 32 * d, 32 * d, si, ci, delta,
+                    octant_only=True)
+    t_cpu = time.perf_counter() - t0
+    cpu_rate = m_ref * (2 * cpu_images + 1) ** 2 / t_cpu
+    req = native.Request(shape=shape, d=d, lam=lam, rho=np.full(int(np.prod(shape)), 0.5), n_k=(0.0, 0.0, 1.0),
+                         n_e0=(0.0, 1.0, 0.0), method="FCD", images=50, period=(32, 32))
+    native.check(native.lib().tdda_trim_caches())
+    p = native.Problem(req, precision="fp32_refined", tables=tables)
+    setup_ms = p.operator_setup_ms()
+    del p
+    gpu_rate = m_ref * 101 ** 2 / (setup_ms * 1e-3)
+    return {"unit": "Green evaluations/s", "cpu": round(cpu_rate, 1), "cpu_threads": threads,
+            "cpu_sample": f"oracle C lattice sum, +-{cpu_images} images, {t_cpu:.2f} s",
+            "gpu": round(gpu_rate, 1), "gpu_setup_ms": round(setup_ms, 3), "gpu_images": 50}
 
 
 PROBE_PER_SHAPE = 8
@@ -212,6 +270,79 @@ def roofline_record(ms5):
             "passes": passes}
 
 
+C4_SHAPE = (128, 128, 64)
+C4_LAMS = (450.0, 550.0, 650.0, 750.0)
+
+
+def smooth_density(shape, sigma, seed):
+    """rho = clip(0.5 + 0.25 f / std f, 0, 1), f = Gaussian-filtered white noise
+    (periodic in x and y, nearest-value edges in z), flattened x-fastest."""
+    from scipy.ndimage import gaussian_filter
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.standard_normal((nz, ny, nx)), sigma=sigma, mode=("nearest", "wrap", "wrap"))
+    return np.clip(0.5 + 0.25 * f / f.std(), 0.0, 1.0).reshape(-1)
+
+
+def multi_rhs_record(native, reps):
+    """128x128x64 dipoles (FFT 256x256x128), d = 8 nm, smooth density (sigma
+    3), 8 right-hand sides = 4 wavelengths x 2 polarisations, 2 per operator.
+    The eight solves run as one batch to 1e-5, refined (field-error target
+    1e-5) and FP64, each with and without the Jacobi preconditioner; the
+    batched c64 matvec is timed per pass with B_mv = 8 (2n + 12 M) + 8 * 6 M /
+    R_op per right-hand side, R_op = 2."""
+    rho = smooth_density(C4_SHAPE, 3.0, seed=5)
+    reqs = [native.Request(shape=C4_SHAPE, d=8.0, lam=l, rho=rho, n_k=(0.0, 0.0, 1.0), n_e0=e)
+            for l in C4_LAMS for e in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))]
+    out = {}
+    for prec in ("fp32_refined", "fp64"):
+        for pc in ("jacobi", "none"):
+            probs = [native.Problem(r, precision=prec, precond=pc) for r in reqs]
+            t0 = time.perf_counter()
+            infos = native.solve(probs, tol=1e-5)
+            dt = time.perf_counter() - t0
+            rec = {"seconds": round(dt, 4), "solves_per_s": round(len(reqs) / dt, 3),
+                   "iterations": [i["iterations"] for i in infos],
+                   "converged": sum(1 for i in infos if i["status"] == "converged")}
+            if prec == "fp32_refined":
+                rec["err_est_max"] = max(i["err_est"] for i in infos)
+            out[f"{prec}_{pc}"] = rec
+            if prec == "fp32_refined" and pc == "jacobi":
+                ms5 = native.time_passes(probs, c64=True, reps=reps)
+                n = 3 * int(np.prod(C4_SHAPE))
+                M = fft_len(C4_SHAPE[0]) * fft_len(C4_SHAPE[1]) * fft_len(C4_SHAPE[2])
+                b_mv = len(reqs) * (8 * (2 * n + 12 * M) + 8 * 6 * M / 2)
+                t = float(sum(ms5)) * 1e-3
+                out["matvec"] = {"ms": round(t * 1e3, 4), "pass_ms": [round(float(x), 4) for x in ms5],
+                                 "algorithmic_bytes": int(b_mv), "GBs": round(b_mv / t / 1e9, 1),
+                                 "frac": round(b_mv / t / 1e9 / HBM_PEAK_GBS, 4)}
+            del probs
+    out["speedup_refined_vs_fp64_batched_jacobi"] = round(
+        out["fp64_jacobi"]["seconds"] / out["fp32_refined_jacobi"]["seconds"], 3)
+    native.check(native.lib().tdda_trim_caches())
+    return out
+
+
+def copy_bandwidth():
+    """Device-to-device copy of 256 MiB timed with HIP events (read + write bytes / time)."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * (256 << 20) * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +354,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=4)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 128x128x64 multi-RHS record")
+    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "none"],
+                    help="preconditioner of the refined server (the FP64 single-solve path is timed with both)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the batched pass timing and a 256 MiB copy (for counter runs)")
     ap.add_argument("--saturation-only", action="store_true",
@@ -241,63 +375,111 @@ def main():
                           "roofline": roofline_record(ms5)}), flush=True)
         return
 
-    reqs = make_requests(native, a.requests)
-    print(f"# {len(reqs)} requests generated", file=sys.stderr, flush=True)
+    # Every step serves its own requests (fresh seeds): wavelengths are
+    # continuous, so no step finds its Green's spectra in the operator cache
+    # and every request pays its set-up inside the timed region.
+    R = a.requests
 
-    # warmup: a short stream (plans, twiddles, allocator, code objects)
-    for _ in range(a.warmup):
-        run_stream(native, reqs[:min(48, len(reqs))], "fp32_refined", a.slots, a.chunk)
+    # warmup: short streams of requests outside the timed seeds (plans,
+    # twiddles, allocator, code objects)
+    for w in range(a.warmup):
+        run_stream(native, make_requests(native, min(48, R), seed0=900000 + 48 * w), "fp32_refined", a.slots,
+                   a.chunk, precond=a.precond)
     print("# warmup done", file=sys.stderr, flush=True)
 
     # timed steps: saturation throughput of the refined batched server
-    walls, res_sat, stats_sat = [], None, None
-    for _ in range(a.steps):
-        w, res_sat, stats_sat, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk)
+    walls, hits, res_steps, stats_sat = [], 0, [], None
+    reqs0 = None
+    for k in range(a.steps):
+        reqs = make_requests(native, R, seed0=1000 + R * k)
+        if k == 0:
+            reqs0 = reqs
+        w, res, stats_sat, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk, precond=a.precond)
         walls.append(w)
-        print(f"# step {len(walls)}: {len(reqs) / w:.1f} solves/s", file=sys.stderr, flush=True)
+        res_steps.append(res)
+        hits += sum(1 for r in res.values() if r["operator_cached"])
+        print(f"# step {k + 1}: {R / w:.1f} solves/s", file=sys.stderr, flush=True)
     total = sum(walls)
-    value = a.steps * len(reqs) / total
-    conv = sum(1 for r in res_sat.values() if r["status"] == "converged")
+    value = a.steps * R / total
+    res_all = [r for res in res_steps for r in res.values()]
+    conv = sum(1 for r in res_all if r["status"] == "converged")
     if a.saturation_only:
-        print(json.dumps({"saturation_solves_per_s": round(value, 3), "converged": conv,
+        print(json.dumps({"saturation_solves_per_s": round(value, 3), "converged": conv, "operator_cache_hits": hits,
                           "server": stats_sat}), flush=True)
         return
 
-    # Poisson arrivals at load x saturation -> latency
+    # Poisson arrivals at load x saturation -> latency (fresh requests again)
     rate = a.load * value
     rng = np.random.default_rng(2)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
     arrivals -= arrivals[0]
-    w_p, res_p, stats_p, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk, arrivals=arrivals)
+    reqs_p = make_requests(native, R, seed0=1000 + R * a.steps)
+    w_p, res_p, stats_p, _ = run_stream(native, reqs_p, "fp32_refined", a.slots, a.chunk, arrivals=arrivals,
+                                        precond=a.precond)
     lat = [(r["t_done"] - r["t_submit"]) * 1e3 for r in res_p.values()]
     print(f"# poisson stream done: p50 {percentile(lat, 0.5):.2f} ms", file=sys.stderr, flush=True)
 
-    # plain FP64 single-solve GPU path on a subset, plus field agreement
-    sub = reqs[:min(a.fp64_requests, len(reqs))]
-    w64, res64, _, f64 = run_stream(native, sub, "fp64", 1, 8, fields=True)
-    _, res32s, _, f32 = run_stream(native, sub, "fp32_refined", a.slots, a.chunk, fields=True)
-    fp64_rate = len(sub) / w64
-    e_err = max(float(np.linalg.norm(f32[i] - f64[i]) / np.linalg.norm(f64[i])) for i in f64)
-    print(f"# fp64 single-solve: {fp64_rate:.2f} solves/s", file=sys.stderr, flush=True)
+    # field accuracy and the FP64 single-solve path on the first requests of
+    # step 1: reference fields from a tight FP64 solve (batched, tol 1e-10)
+    sub = reqs0[:min(a.fp64_requests, R)]
+    _, res_t, _, f_tight = run_stream(native, sub, "fp64", len(sub), 16, fields=True, tol=1e-10, precond="jacobi")
+    _, res32s, _, f32 = run_stream(native, sub, "fp32_refined", a.slots, a.chunk, fields=True, precond=a.precond)
+    err32 = [rel_err(f32[i], f_tight[i]) for i in f_tight]
+    est = [res32s[i]["err_est"] for i in f_tight]
+    # the FP64 single-solve path (one request at a time, FP64 transforms and
+    # vectors, the reference's recursive-residual stopping rule), without and
+    # with the preconditioner, at tol 1e-5 and at the tolerance that gives the
+    # refined path's field accuracy (field error ~ 2 x residual here)
+    tol_eq = 1e-5 / 3.0
+    fp64 = {}
+    for pc in ("none", "jacobi"):
+        for tol in (1e-5, tol_eq):
+            w64, res64, _, f64 = run_stream(native, sub, "fp64", 1, 8, fields=True, tol=tol, precond=pc)
+            errs = [rel_err(f64[i], f_tight[i]) for i in f_tight]
+            fp64[(pc, tol)] = {"solves_per_s": round(len(sub) / w64, 3), "tol": round(tol, 9),
+                               "iterations_p50": percentile([r["iterations"] for r in res64.values()], 0.5),
+                               "field_err_max": max(errs), "field_err_p50": percentile(errs, 0.5),
+                               "_iters": {i: r["iterations"] for i, r in res64.items()}}
+    fp64_rate = fp64[("none", 1e-5)]["solves_per_s"]
+    print(f"# fp64 single-solve: {fp64_rate:.2f} solves/s (jacobi {fp64[('jacobi', 1e-5)]['solves_per_s']:.2f}); "
+          f"refined field error max {max(err32):.3e}", file=sys.stderr, flush=True)
+    # the refined server without preconditioning on one step's requests
+    w_np, res_np, _, _ = run_stream(native, reqs0, "fp32_refined", a.slots, a.chunk, precond="none")
 
     # roofline of the batched matvec (one launch per pass over the probe batch)
     ms5, ms10, probs = roofline_probe(native, 30)
     roof = roofline_record(ms5)
     roof["iteration_pass_ms"] = [round(float(x), 5) for x in ms10]
     del probs
+    native.check(native.lib().tdda_trim_caches())
+    copy_gbs = copy_bandwidth()
 
-    cpu = None
+    c4 = None if a.no_c4 else multi_rhs_record(native, 10)
+    if c4:
+        print(f"# multi-RHS 128x128x64: {c4['fp32_refined_jacobi']['solves_per_s']} solves/s refined", file=sys.stderr,
+              flush=True)
+
+    cpu, lattice = None, None
+    host = host_cpu()
     if not a.no_cpu:
+        import dda_oracle as O
         # iteration counts: the FP64 GPU run where it covered the request
         # (same recurrence as the oracle), else the refined run's count
-        its = [res64[i]["iterations"] if i in res64 else res_sat[i]["iterations"] for i in range(len(reqs))]
-        rate_cpu, sample_s, desc = cpu_baseline(reqs, its)
-        cpu = {"value": round(rate_cpu, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle (numpy+scipy.fft, workers=1) timed for {sample_s:.1f} s on the first "
-                         f"request of each box shape (set-up + BiCGSTAB iterations: {desc}); scaled to the "
-                         f"{len(reqs)}-request stream by each request's FP64 iteration count"}
+        its64 = fp64[("none", 1e-5)]["_iters"]
+        its = [its64[i] if i in its64 else res_np[i]["iterations"] for i in range(R)]
+        threads = min(16, host["nproc"] or 1)
+        rate1, s1, desc1 = cpu_baseline(reqs0, its, workers=1)
+        rate_n, s_n, desc_n = cpu_baseline(reqs0, its, workers=threads)
+        cpu = {"value": round(rate_n, 5), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle (numpy + scipy.fft with {threads} FFT threads) timed for {s_n:.1f} s on the "
+                         f"first request of each box shape (set-up + BiCGSTAB iterations: {desc_n}); scaled to the "
+                         f"{R}-request stream by each request's FP64 iteration count",
+               "single_thread": {"value": round(rate1, 5), "cores": 1, "sample_s": round(s1, 1),
+                                 "detail": desc1},
+               "host": host}
+        lattice = lattice_sum_rates(native, O.sici_tables(), int(os.environ.get("OMP_NUM_THREADS", threads)))
 
-    iters = [r["iterations"] for r in res_sat.values()]
+    iters = [r["iterations"] for r in res_all]
     out = {
         "metric": "DDA solves/sec at 1e-5 residual, FP32-refined batched vs FP64; p50/p99 ms",
         "value": round(value, 3),
@@ -311,21 +493,40 @@ def main():
         "vs_baseline": None,
         "dtype": "c64+f64-refine",
         "data": "synthetic",
-        "config": {"workload": f"serving stream of {len(reqs)} DDA requests (boxes 16^3..64x64x32, d=15nm, "
-                               "lambda U[400,800]nm, polar angle U[0,30deg], random polarisation, density "
-                               "U[0,1), LDR, non-periodic), solved to 1e-5 true residual",
+        "config": {"workload": f"serving stream of {R} DDA requests per step, fresh seeds every step (boxes "
+                               "16^3..64x64x32, d=15nm, lambda U[400,800]nm, polar angle U[0,30deg], random "
+                               "polarisation, density U[0,1), LDR, non-periodic), each solved to 1e-5 true residual "
+                               "and 1e-5 estimated field error (right-Jacobi preconditioned BiCGSTAB unless precond "
+                               "says otherwise); Green's set-up inside the timed region",
                    "slots": a.slots, "chunk": a.chunk, "poisson_load": a.load},
+        "operator_cache_hits": hits,
         "p50_ms": round(percentile(lat, 0.5), 3),
         "p99_ms": round(percentile(lat, 0.99), 3),
         "poisson_rate_per_s": round(rate, 2),
-        "poisson_solves_per_s": round(len(reqs) / w_p, 3),
+        "poisson_solves_per_s": round(R / w_p, 3),
         "converged": conv,
         "iterations_p50": percentile(iters, 0.5),
-        "fp64_single_solve": {"solves_per_s": round(fp64_rate, 3), "requests": len(sub),
-                              "iterations_p50": percentile([r["iterations"] for r in res64.values()], 0.5)},
+        "err_est_p50": percentile([r["err_est"] for r in res_all], 0.5),
+        "err_est_max": max(r["err_est"] for r in res_all),
+        "precond": a.precond,
+        "field_accuracy": {
+            "requests": len(sub), "reference": "FP64 solve to 1e-10 relative residual",
+            "refined_max": max(err32), "refined_p50": percentile(err32, 0.5),
+            "refined_within_1e-5": bool(max(err32) <= 1e-5),
+            "refined_estimate_over_actual_min": min(e / a_ for e, a_ in zip(est, err32)),
+            "refined_estimate_over_actual_max": max(e / a_ for e, a_ in zip(est, err32))},
+        "fp64_single_solve": {f"{pc}_tol_{tol:.3g}": {k: v for k, v in rec.items() if k != "_iters"}
+                              for (pc, tol), rec in fp64.items()},
         "speedup_vs_fp64_single": round(value / fp64_rate, 3),
-        "field_rel_err_refined_vs_fp64_max": e_err,
+        "speedup_vs_fp64_single_jacobi": round(value / fp64[("jacobi", 1e-5)]["solves_per_s"], 3),
+        "speedup_vs_fp64_single_equal_accuracy": round(value / fp64[("none", tol_eq)]["solves_per_s"], 3),
+        "speedup_vs_fp64_single_jacobi_equal_accuracy": round(value / fp64[("jacobi", tol_eq)]["solves_per_s"], 3),
+        "refined_unpreconditioned": {"solves_per_s": round(R / w_np, 3), "requests": R,
+                                     "iterations_p50": percentile([r["iterations"] for r in res_np.values()], 0.5)},
         "server": {k: stats_sat[k] for k in ("chunks", "rebuilds", "launches", "max_active", "peak_bytes")},
+        "copy_GBs": round(copy_gbs, 1),
+        "multi_rhs_128x128x64": c4,
+        "lattice_sum": lattice,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -26,6 +26,9 @@ extern "C" {
 #define TDDA_PREC_FP64 0          /* complex128 transforms and Krylov vectors */
 #define TDDA_PREC_FP32_REFINED 1  /* complex64 transforms/Krylov, FP64 reliable updates */
 
+#define TDDA_PRECOND_NONE 0    /* the reference's plain BiCGSTAB (DDAModel.cpp:899-1009) */
+#define TDDA_PRECOND_JACOBI 1  /* right Jacobi: BiCGSTAB on A diag(1/alpha)^-1, same residual */
+
 #define TDDA_GRAD_REFERENCE 0  /* one-sided FD partials as DDAModel.cpp:471-578 */
 #define TDDA_GRAD_EXACT 1      /* closed-form partials, same adjoint solve */
 
@@ -174,6 +177,14 @@ int tdda_problem_destroy(tdda_problem* p);
 int tdda_problem_set_density(tdda_problem* p, const double* rho, int on_device);
 /* solve all problems together (one batched launch per pass); infos: n entries */
 int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_solve_info* infos);
+/* Preconditioner of the problem's later solves (TDDA_PRECOND_*); no reference
+ * counterpart: the reference's BiCGSTAB is unpreconditioned.  Right
+ * preconditioning keeps the residual, hence the stopping rule, unchanged. */
+int tdda_problem_set_preconditioner(tdda_problem* p, int precond);
+/* FFT grid (Lx, Ly, Lz), Green's set-up time of the problem's operator in ms
+ * (lattice sum, DFTs, spectrum layout; AProductCore::AProductCore,
+ * src/AProductCore.cpp:12-298) and whether it came from the operator cache. */
+int tdda_problem_grid(tdda_problem* p, int* lxyz, double* setup_ms, int* cached);
 /* E = al P 3/(eps+2): 3N complex, to host or device memory */
 int tdda_problem_field(tdda_problem* p, double* E, int on_device);
 /* polarisation P: 3N complex, to host or device memory */
@@ -215,6 +226,7 @@ typedef struct {
     int chunk;           /* iterations between host checks */
     int keep_fields;     /* keep E of finished requests for tdda_server_take_field */
     double err_tol;      /* refined: relative field-error target; 0 = tol, < 0 = true residual only */
+    int precond;         /* TDDA_PRECOND_* for every solve */
 } tdda_server_opts;
 
 typedef struct {

@@ -213,7 +213,7 @@ def test_multi_rhs_128x128x64(native, O):
     infos = native.solve(probs, tol=1e-5)
     _log(f"[gpu] multi-RHS: {[(i['status'], i['iterations'], round(i['err_est'], 8)) for i in infos]}")
     for i in infos:
-        assert i["status"] == "converged" and i["error"] <= 1e-5 and 1.5 * i["err_est"] <= 1e-5
+        assert i["status"] == "converged" and i["error"] <= 1e-5 and 2.0 * i["err_est"] <= 1e-5
     r = _true_residual(box, probs[2].polarization())
     _log(f"[gpu] multi-RHS: oracle true residual of the 550 nm x-polarised solve {r:.3e}")
     assert r <= 1.0001e-5

@@ -68,17 +68,18 @@ def test_problem_fcd_periodic_matches_oracle(native, tables):
     assert _rel(p.field(), _oracle_field(O, req, tables)) < 1e-8
 
 
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
-def test_batch_equals_alone(native, tables, precision):
-    _log(f"[gpu] batch vs alone {precision}")
+def test_batch_equals_alone(native, tables, precision, precond):
+    _log(f"[gpu] batch vs alone {precision} {precond}")
     reqs = [_request(native, s, seed=10 + i, lam=450 + 40 * i, theta=0.2 * i)
             for i, s in enumerate([(16, 16, 16), (24, 24, 24), (12, 20, 8)])]
     alone = []
     for r in reqs:
-        p = native.Problem(r, precision=precision, tables=tables)
+        p = native.Problem(r, precision=precision, tables=tables, precond=precond)
         native.solve([p])
         alone.append(p.field())
-    ps = [native.Problem(r, precision=precision, tables=tables) for r in reqs]
+    ps = [native.Problem(r, precision=precision, tables=tables, precond=precond) for r in reqs]
     infos = native.solve(ps)
     for p, a, inf in zip(ps, alone, infos):
         assert inf["status"] == "converged"
@@ -151,3 +152,71 @@ def test_server_admission_by_memory(native, tables):
     srv.close()
     assert len(done) == 3 and all(d["status"] == "converged" for d in done)
     assert st["max_active"] == 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_jacobi_preconditioned_solves(native, tables, precision):
+    """Right Jacobi preconditioning (BiCGSTAB on A diag(al)^-1, P = Y / al):
+    the residual is the original system's, so a solve to 1e-12 gives the
+    oracle's field, and a refined solve to 1e-5 meets the field bound, in
+    far fewer iterations than the plain recurrence."""
+    import dda_oracle as O
+    _log(f"[gpu] jacobi {precision}")
+    for i, (shape, method, images) in enumerate([((16, 16, 16), "LDR", 0), ((8, 8, 4), "FCD", 4)]):
+        req = _request(native, shape, seed=300 + i, lam=520.0 + 40 * i, method=method, images=images,
+                       theta=0.25, phi=0.7, psi=1.3)
+        e_ref = _oracle_field(O, req, tables)
+        if precision == "fp64":
+            p = native.Problem(req, precision="fp64", tables=tables, precond="jacobi")
+            info = native.solve([p], tol=1e-12)[0]
+            assert info["status"] == "converged"
+            assert _rel(p.field(), e_ref) < 1e-9
+        its = {}
+        for pc in ("none", "jacobi"):
+            p = native.Problem(req, precision=precision, tables=tables, precond=pc)
+            info = native.solve([p], tol=1e-5)[0]
+            its[pc] = info["iterations"]
+            err = _rel(p.field(), e_ref)
+            _log(f"[gpu] {shape} {method} {precision} {pc}: {info}; E err {err:.3e}")
+            assert info["status"] == "converged"
+            if precision == "fp32_refined":
+                assert err <= 1e-5
+            else:
+                assert err <= 5e-5
+        assert its["jacobi"] < its["none"]
+
+
+def test_server_jacobi_mix_equals_alone_and_oracle(native, tables):
+    import dda_oracle as O
+    _log("[gpu] jacobi server mix vs alone")
+    shapes = [(16, 16, 16), (24, 24, 24), (32, 32, 16), (20, 12, 10)]
+    reqs = [_request(native, s, seed=400 + i, lam=430 + 90 * i, theta=0.1 * i, phi=0.4 * i, psi=0.9 * i)
+            for i, s in enumerate(shapes)]
+    srv = native.Server(precision="fp32_refined", max_slots=8, chunk=2, precond="jacobi")
+    for i, r in enumerate(reqs):
+        srv.submit(i, r)
+    import time
+    got = {}
+    t_end = time.time() + 60
+    while len(got) < len(reqs) and time.time() < t_end:
+        for res in srv.poll():
+            got[res["id"]] = res
+        time.sleep(0.002)
+    assert len(got) == len(reqs)
+    mix = {i: srv.field(i) for i in got}
+    srv.close()
+    for i, r in enumerate(reqs):
+        s1 = native.Server(precision="fp32_refined", max_slots=1, chunk=3, precond="jacobi")
+        s1.submit(0, r)
+        res = []
+        while not res:
+            res = s1.poll()
+            time.sleep(0.001)
+        alone = s1.field(0)
+        s1.close()
+        assert got[i]["status"] == "converged" and res[0]["iterations"] == got[i]["iterations"]
+        assert _rel(mix[i], alone) <= 1e-14
+        if i in (0, 3):
+            err = _rel(mix[i], _oracle_field(O, r, tables))
+            _log(f"[gpu] jacobi served {i}: {got[i]['iterations']} its, E err {err:.3e}, estimate {got[i]['err_est']:.3e}")
+            assert err <= 1e-5

@@ -350,6 +350,29 @@ int tdda_problems_solve(tdda_problem** ps, int n, int maxit, double tol, tdda_so
     });
 }
 
+int tdda_problem_set_preconditioner(tdda_problem* p, int precond) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p, tdda::ERR_INVALID, "null problem");
+        TDDA_REQUIRE(precond == TDDA_PRECOND_NONE || precond == TDDA_PRECOND_JACOBI, tdda::ERR_INVALID,
+                     "unknown preconditioner");
+        p->p->sys->set_jacobi(precond == TDDA_PRECOND_JACOBI);
+    });
+}
+
+int tdda_problem_grid(tdda_problem* p, int* lxyz, double* setup_ms, int* cached) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p, tdda::ERR_INVALID, "null problem");
+        const tdda::Operator& op = *p->p->op;
+        if (lxyz) {
+            lxyz[0] = op.lx;
+            lxyz[1] = op.ly;
+            lxyz[2] = op.lz;
+        }
+        if (setup_ms) *setup_ms = op.setup_ms;
+        if (cached) *cached = p->p->op_hit ? 1 : 0;
+    });
+}
+
 int tdda_problem_field(tdda_problem* p, double* E, int on_device) {
     return guarded([&] {
         TDDA_REQUIRE(p && p->p && E, tdda::ERR_INVALID, "null argument");
@@ -465,6 +488,9 @@ int tdda_server_create(const tdda_server_opts* o, const double* si, const double
         so.max_slots = o->max_slots;
         so.chunk = o->chunk;
         so.keep_fields = o->keep_fields != 0;
+        TDDA_REQUIRE(o->precond == TDDA_PRECOND_NONE || o->precond == TDDA_PRECOND_JACOBI, tdda::ERR_INVALID,
+                     "unknown preconditioner");
+        so.jacobi = o->precond == TDDA_PRECOND_JACOBI;
         tdda::SiciDevice t = tdda::sici_tables(si, ci, si ? nsici : 0, delta);
         *out = new tdda_server{new tdda::Server(so, t)};
     });

@@ -41,6 +41,7 @@ template <> struct Tile<float> {
     __device__ static int txf(const GridJob& J) { return J.txf32; }
     __device__ static const float2* green(const GridJob& J) { return J.green32; }
     __device__ static const float2* al(const GridJob& J) { return J.al32; }
+    __device__ static const float2* ial(const GridJob& J) { return J.ial32; }
 };
 template <> struct Tile<double> {
     __device__ static int bx(const GridJob& J) { return J.bx64; }
@@ -51,6 +52,7 @@ template <> struct Tile<double> {
     __device__ static int txf(const GridJob& J) { return J.txf64; }
     __device__ static const double2* green(const GridJob& J) { return J.green64; }
     __device__ static const double2* al(const GridJob& J) { return J.al64; }
+    __device__ static const double2* ial(const GridJob& J) { return J.ial64; }
 };
 
 // GATE_ACTIVE: a running job not waiting for its reliable update;
@@ -253,13 +255,11 @@ __device__ inline double2 apply_pending(const GridJob& J, unsigned vi, const XfS
     return Pn;
 }
 
+// The vector a mode feeds to the transform (before the Jacobi scaling), with
+// the mode's vector updates applied.
 template <typename C, typename KC, int MODE>
-__device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc, double (&xacc)[XF_NACC]) {
-    if (vi < 0) return mk<C>(0.0, 0.0);
-    const unsigned i = (unsigned)vi;
-    if (MODE == XF_PLAIN) {
-        return gload<C>(J.xin, i);
-    } else if (MODE == XF_SOL) {
+__device__ inline C xf_vector(const GridJob& J, unsigned i, const XfScalars& sc, double (&xacc)[XF_NACC]) {
+    if (MODE == XF_SOL) {
         double2 delta = make_double2(0.0, 0.0);
         double2 Pn;
         if (sc.pending) {
@@ -269,6 +269,11 @@ __device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc, do
             Pn = fold_solution<KC>(J, i, make_double2(0.0, 0.0), &delta);
         } else {
             Pn = gload<double2>(J.P, i);
+        }
+        if (J.jacobi) {   // P holds Y = al P: the polarisation and its change are Y / al
+            const double2 ia = gload<double2>(J.ial64, i / 3u);
+            Pn = dcmul(Pn, ia);
+            delta = dcmul(delta, ia);
         }
         if (J.partial_xf != nullptr) {
             const double2 f = J.efac != nullptr ? to_d(gload<float2>(J.efac, i / 3u)) : make_double2(1.0, 0.0);
@@ -305,6 +310,21 @@ __device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc, do
     }
 }
 
+// X forward input of element vi (< 0: outside the geometry): the mode's
+// vector, times 1/al under Jacobi preconditioning.
+template <typename C, typename KC, int MODE>
+__device__ inline C xf_element(const GridJob& J, int vi, const XfScalars& sc, double (&xacc)[XF_NACC]) {
+    if (vi < 0) return mk<C>(0.0, 0.0);
+    const unsigned i = (unsigned)vi;
+    if (MODE == XF_PLAIN) return gload<C>(J.xin, i);
+    const C u = xf_vector<C, KC, MODE>(J, i, sc, xacc);
+    if (MODE != XF_SOL && J.jacobi) {   // the transform sees x / al (XF_SOL scales inside)
+        using Real = decltype(u.x);
+        return cmul(u, gload<C>(Tile<Real>::ial(J), i / 3u));
+    }
+    return u;
+}
+
 // dipole vector index of box cell (row q = z*ny + y, x) component c, or -1
 __device__ inline int vec_index(const GridJob& J, int q, int x, int c) {
     const int cell = q * J.nx + x;
@@ -326,7 +346,8 @@ template <typename C, typename KC, int MODE>
 __device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __restrict__ al,
                                   double (&acc)[XI_NACC]) {
     const unsigned i = (unsigned)vi;
-    const C a = gload<C>(al, i / 3u);
+    // Jacobi: the operator is A diag(al)^-1, whose diagonal part al (x / al) is x
+    const C a = (MODE != XI_PLAIN && J.jacobi) ? mk<C>(1.0, 0.0) : gload<C>(al, i / 3u);
     if (MODE == XI_PLAIN) {
         const C u = gload<C>(J.xin, i);
         gstore<C>(J.yout, i, cadd(conv, cmul(a, u)));

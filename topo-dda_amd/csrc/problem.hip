@@ -39,13 +39,17 @@ __device__ inline double2 inv_alpha(const SetupParams& P, double2 eps) {
 }
 
 __global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, double2* __restrict__ al64,
-                                float2* __restrict__ al32, double2* __restrict__ rhs, float2* __restrict__ efac) {
+                                float2* __restrict__ al32, double2* __restrict__ ial64, float2* __restrict__ ial32,
+                                double2* __restrict__ rhs, float2* __restrict__ efac) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const double2 eps = eps_of(P, rho[i]);
     const double2 al = inv_alpha(P, eps);
     al64[i] = al;
     al32[i] = make_float2((float)al.x, (float)al.y);
+    const double2 ia = ddiv(make_double2(1.0, 0.0), al);   // alpha (Jacobi preconditioning)
+    ial64[i] = ia;
+    ial32[i] = make_float2((float)ia.x, (float)ia.y);
     if (efac != nullptr) {   // E = f P, f = al 3/(eps + 2) (DDAModel.cpp:1185-1192)
         const double2 f = ddiv(al, make_double2((eps.x + 2.0) / 3.0, eps.y / 3.0));
         efac[i] = make_float2((float)f.x, (float)f.y);
@@ -138,7 +142,7 @@ void Problem::set_density(const double* r, bool on_device, hipStream_t st) {
         rho.upload(r, ndip, st);
     SetupParams P = params_of(*this);
     hipLaunchKernelGGL(k_problem_setup, dim3((ndip + 255) / 256), dim3(256), 0, st, P, rho.get(), sys->al64.get(),
-                       sys->al32.get(), sys->rhs.get(), sys->efac.get());
+                       sys->al32.get(), sys->ial64.get(), sys->ial32.get(), sys->rhs.get(), sys->efac.get());
     TDDA_HIP(hipGetLastError());
     sys->job.efac = sys->efac.get();
 }
@@ -155,7 +159,7 @@ size_t Problem::estimate_bytes(const ProblemDesc& d, bool refined, bool need_ope
     const size_t n = (size_t)d.nx * d.ny * d.nz;
     const size_t kb = refined ? sizeof(float2) : sizeof(double2);
     size_t b = (size_t)3 * d.nz * ly * lx * sizeof(double2);   // work grid
-    b += 6 * 3 * n * kb + 3 * 3 * n * sizeof(double2) + n * (sizeof(double2) + sizeof(float2) + sizeof(double));
+    b += 6 * 3 * n * kb + 3 * 3 * n * sizeof(double2) + n * (2 * sizeof(double2) + 3 * sizeof(float2) + sizeof(double));
     b += ((size_t)d.ny * d.nz + 3 * n / 1024 + 8) * 6 * sizeof(double);
     if (need_operator) {
         const size_t h = (size_t)(lx / 2 + 1) * (ly / 2 + 1) * (lz / 2 + 1);

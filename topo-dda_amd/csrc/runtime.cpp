@@ -356,7 +356,7 @@ size_t System::estimate_bytes(const Operator& op, int ndip, bool refined) {
     size_t b = grid_elems(op) * sizeof(double2);
     b += 6 * n3 * ks;
     b += 3 * n3 * sizeof(double2);   // P, rhs, scratch64
-    b += ndip * (sizeof(double2) + sizeof(float2));
+    b += 2 * ndip * (sizeof(double2) + sizeof(float2));
     b += (size_t)std::max<long>(n3 / (FFT_THREADS * UPD_EPT) + 1, (long)op.ny * op.nz) * 6 * sizeof(double);
     if (refined) b += (size_t)op.ny * op.nz * XF_NACC * sizeof(double) + ndip * sizeof(float2);
     return b;
@@ -375,6 +375,8 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     scratch64.alloc(n3);
     al64.alloc(ndip);
     al32.alloc(ndip);
+    ial64.alloc(ndip);
+    ial32.alloc(ndip);
     P.zero(st);
 
     std::memset(&job, 0, sizeof job);
@@ -402,6 +404,9 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.green64 = o.green64.get();
     job.al32 = al32.get();
     job.al64 = al64.get();
+    job.ial32 = ial32.get();
+    job.ial64 = ial64.get();
+    job.jacobi = 0;
     job.grid = grid.get();
     char* k = kvec.get();
     job.r = k;
@@ -421,25 +426,46 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
 
 size_t System::bytes() const {
     return grid.bytes() + kvec.bytes() + P.bytes() + rhs.bytes() + scratch64.bytes() + al64.bytes() +
-           al32.bytes() + partial.bytes() + efac.bytes() + box_map.bytes();
+           al32.bytes() + ial64.bytes() + ial32.bytes() + partial.bytes() + efac.bytes() + box_map.bytes();
 }
 
 namespace {
-__global__ void k_alpha32(const double2* __restrict__ a, float2* __restrict__ b, long n) {
+// c64 copy of 1/alpha and alpha itself in both precisions
+__global__ void k_alpha_derived(const double2* __restrict__ a, float2* __restrict__ a32, double2* __restrict__ ia64,
+                                float2* __restrict__ ia32, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+    if (i >= n) return;
+    const double2 v = a[i];
+    a32[i] = make_float2((float)v.x, (float)v.y);
+    const double2 w = ddiv(make_double2(1.0, 0.0), v);
+    ia64[i] = w;
+    ia32[i] = make_float2((float)w.x, (float)w.y);
+}
+
+__global__ void k_scale_solution(double2* __restrict__ P, const double2* __restrict__ f, long n3) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) P[i] = dcmul(P[i], f[i / 3]);
 }
 }  // namespace
 
 void System::set_alpha(const double2* al_dip_host, hipStream_t st) {
     al64.upload(al_dip_host, ndip, st);
-    hipLaunchKernelGGL(k_alpha32, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(), (long)ndip);
+    hipLaunchKernelGGL(k_alpha_derived, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(),
+                       ial64.get(), ial32.get(), (long)ndip);
     TDDA_HIP(hipGetLastError());
 }
 
 void System::set_alpha_device(const double2* al_dip_dev, hipStream_t st) {
     TDDA_HIP(hipMemcpyAsync(al64.get(), al_dip_dev, ndip * sizeof(double2), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_alpha32, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(), (long)ndip);
+    hipLaunchKernelGGL(k_alpha_derived, dim3((ndip + 255) / 256), dim3(256), 0, st, al64.get(), al32.get(),
+                       ial64.get(), ial32.get(), (long)ndip);
+    TDDA_HIP(hipGetLastError());
+}
+
+void System::scale_solution(bool to_y, hipStream_t st) {
+    const long n3 = 3L * ndip;
+    hipLaunchKernelGGL(k_scale_solution, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, P.get(),
+                       to_y ? al64.get() : ial64.get(), n3);
     TDDA_HIP(hipGetLastError());
 }
 
@@ -703,6 +729,10 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
         launch_matvec(l64, tw, false, XF_PLAIN, XI_PLAIN, GATE_ALWAYS, st);
     }
     launch_init(b.ls, opt.warm, st);
+    // preconditioned systems iterate on Y = al P
+    if (opt.warm)
+        for (System* s : systems)
+            if (s->jacobi()) s->scale_solution(true, st);
     const bool c64 = opt.refined;
     std::vector<KScalars> snap(nj);
     int chunk = opt.chunk > 0 ? opt.chunk : 4;
@@ -721,7 +751,12 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
             if (snap[j].status == JOB_ACTIVE) all_done = false;
             max_iter = std::max(max_iter, snap[j].iter);
         }
-        if (all_done) break;
+        if (all_done) {
+            for (System* s : systems)
+                if (s->jacobi()) s->scale_solution(false, st);
+            TDDA_HIP(hipStreamSynchronize(st));
+            break;
+        }
         if (opt.chunk <= 0) {
             // grow the chunk while the solve keeps going; bounded by maxit
             chunk = std::min(32, chunk * 2);

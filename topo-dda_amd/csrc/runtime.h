@@ -161,6 +161,13 @@ class System {
     // local-field factor f = al 3/(eps + 2) per dipole (refined systems): the
     // field-error estimate weighs the solution by it (E = f P)
     void set_field_factor(const double2* f_dip_host, hipStream_t st);
+    // right Jacobi preconditioning of later solves (off: the reference's plain
+    // BiCGSTAB, DDAModel.cpp:899-1009)
+    void set_jacobi(bool on) { job.jacobi = on ? 1 : 0; }
+    bool jacobi() const { return job.jacobi != 0; }
+    // P <- P / al (Y of a preconditioned solve back to the polarisation) or
+    // P <- al P (a polarisation to Y)
+    void scale_solution(bool to_y, hipStream_t st);
     void set_rhs(const double2* rhs_host, hipStream_t st);
     void set_solution(const double2* P_host, hipStream_t st);
     void get_solution(double2* P_host, hipStream_t st) const;
@@ -174,8 +181,8 @@ class System {
     DevBuf<char> grid;          // c128-sized work grid (c64 view aliases it)
     DevBuf<char> kvec;          // r, r0, p, v, t, s (+ dP in refined mode)
     DevBuf<double2> P, rhs, scratch64;
-    DevBuf<double2> al64;
-    DevBuf<float2> al32;
+    DevBuf<double2> al64, ial64;   // 1/alpha and alpha per dipole
+    DevBuf<float2> al32, ial32;
     DevBuf<double> partial;     // X inverse partials, then (refined) the X forward ones
     DevBuf<float2> efac;        // refined: local-field factor per dipole
     DevBuf<int> box_map;

@@ -136,6 +136,7 @@ std::vector<int> Server::admit() {
         sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
         sl.prob->set_density(req.rho.data(), false, setup_);
         sl.prob->sys->P.zero(setup_);
+        sl.prob->sys->set_jacobi(opt_.jacobi);
         sl.op_hit = sl.prob->op_hit ? 1 : 0;
         sl.t_admit = now();
         sl.req = std::move(req);
@@ -149,6 +150,10 @@ void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
     if (fin.empty()) return;
     std::vector<DevBuf<double2>> ebuf(fin.size());
     std::vector<std::vector<double2>> eh(fin.size());
+    // preconditioned solves end with Y = al P in P (the batch is gated off
+    // for them, so the next chunk leaves it alone)
+    for (int f : fin)
+        if (slots_[f].prob->sys->jacobi()) slots_[f].prob->sys->scale_solution(false, main_);
     if (opt_.keep_fields) {
         for (size_t k = 0; k < fin.size(); ++k) {
             Problem& p = *slots_[fin[k]].prob;

@@ -27,6 +27,7 @@ struct ServerOptions {
     double tol = 1e-5;
     double ru_delta = 0.1;
     double err_tol = 1e-5;           // refined: field-error target (0: true residual only)
+    bool jacobi = false;             // right Jacobi preconditioning of every solve
     size_t hbm_budget = size_t(200) << 30;
     int max_slots = 64;
     int chunk = 4;                   // iterations between host checks

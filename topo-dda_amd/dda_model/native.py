@@ -13,6 +13,7 @@ import numpy as np
 
 _LIB = None
 PREC = {"fp64": 0, "fp32_refined": 1}
+PRECOND = {"none": 0, "jacobi": 1}
 STATUS = {0: "active", 1: "converged", 2: "breakdown", 3: "maxit", 4: "idle"}
 OBJ = {"integrated": 0, "targets": 1}
 
@@ -34,7 +35,8 @@ class SolveInfo(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("maxit", ctypes.c_int), ("tol", ctypes.c_double),
                 ("ru_delta", ctypes.c_double), ("hbm_budget_gb", ctypes.c_double), ("max_slots", ctypes.c_int),
-                ("chunk", ctypes.c_int), ("keep_fields", ctypes.c_int), ("err_tol", ctypes.c_double)]
+                ("chunk", ctypes.c_int), ("keep_fields", ctypes.c_int), ("err_tol", ctypes.c_double),
+                ("precond", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -66,6 +68,8 @@ def lib():
         L.tdda_problems_solve.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.POINTER(SolveInfo)]
         L.tdda_problem_field.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
+        L.tdda_problem_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
+        L.tdda_problem_set_preconditioner.argtypes = [vp, ctypes.c_int]
         L.tdda_problem_polarization.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
         L.tdda_problem_objective.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp]
         L.tdda_problem_gradient.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
@@ -136,7 +140,9 @@ def _tables(tables):
 class Problem:
     """A device-resident problem (operator from the device cache)."""
 
-    def __init__(self, req: Request, precision="fp32_refined", tables=None):
+    def __init__(self, req: Request, precision="fp32_refined", tables=None, precond="none"):
+        """precond: "none" (the reference's plain BiCGSTAB) or "jacobi" (right
+        Jacobi preconditioning; the residual and the stopping rule are unchanged)."""
         si, ci, n, delta = _tables(tables)
         self._keep = (si, ci)
         h = ctypes.c_void_p()
@@ -145,6 +151,7 @@ class Problem:
                                         _dp(ci) if ci is not None else None, n, delta, ctypes.byref(h)))
         self.h = h
         self.n = int(np.prod(req.shape))
+        check(lib().tdda_problem_set_preconditioner(self.h, PRECOND[precond]))
         self.set_density(req.rho)
 
     def set_density(self, rho):
@@ -155,6 +162,17 @@ class Problem:
             r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
             assert r.size == self.n
             check(lib().tdda_problem_set_density(self.h, r.ctypes.data_as(ctypes.c_void_p), 0))
+
+    def grid(self):
+        """(FFT grid (Lx, Ly, Lz), operator set-up ms, operator taken from the cache)"""
+        lxyz = (ctypes.c_int * 3)()
+        ms = ctypes.c_double(0.0)
+        cached = ctypes.c_int(0)
+        check(lib().tdda_problem_grid(self.h, lxyz, ctypes.byref(ms), ctypes.byref(cached)))
+        return tuple(lxyz), ms.value, bool(cached.value)
+
+    def operator_setup_ms(self):
+        return self.grid()[1]
 
     def field(self):
         out = np.empty(3 * self.n, dtype=np.complex128)
@@ -255,13 +273,14 @@ class Server:
     """Continuous-batching request server (C++ loop thread)."""
 
     def __init__(self, precision="fp32_refined", maxit=100000, tol=1e-5, ru_delta=0.1, hbm_budget_gb=200.0,
-                 max_slots=64, chunk=4, keep_fields=True, tables=None, err_tol=0.0):
+                 max_slots=64, chunk=4, keep_fields=True, tables=None, err_tol=0.0, precond="none"):
         """err_tol: relative field-error target of refined solves (0: tol;
-        negative: stop on the true residual alone)."""
+        negative: stop on the true residual alone).  precond: "none" or
+        "jacobi" for every solve."""
         si, ci, n, delta = _tables(tables)
         self._keep = (si, ci)
         o = ServerOpts(PREC[precision], maxit, tol, ru_delta, hbm_budget_gb, max_slots, chunk, int(keep_fields),
-                       err_tol)
+                       err_tol, PRECOND[precond])
         h = ctypes.c_void_p()
         check(lib().tdda_server_create(ctypes.byref(o), _dp(si) if si is not None else None,
                                        _dp(ci) if ci is not None else None, n, delta, ctypes.byref(h)))
