@@ -79,12 +79,14 @@ def percentile(v, q):
 
 
 def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=False, timeout=600.0, tol=1e-5,
-               err_tol=0.0, precond="none"):
+               err_tol=0.0, precond="none", server=None, id0=0):
     """Submit every request (at its arrival time if given) and wait for all.
-    Returns (wall seconds from first submission to last completion, results,
+    A given server is used and kept open (request ids id0, id0 + 1, ...);
+    otherwise one is made for the stream and closed.  Returns (wall seconds
+    from first submission to last completion, results by request index,
     server statistics, fields or None)."""
-    srv = native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields, tol=tol,
-                        err_tol=err_tol, precond=precond)
+    srv = server or native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields,
+                                  tol=tol, err_tol=err_tol, precond=precond)
     results = {}
     t0 = time.perf_counter()
     nxt = 0
@@ -92,17 +94,18 @@ def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=
     while len(results) < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and (arrivals is None or arrivals[nxt] <= now):
-            srv.submit(nxt, reqs[nxt])
+            srv.submit(id0 + nxt, reqs[nxt])
             nxt += 1
         for r in srv.poll():
-            results[r["id"]] = r
+            results[r["id"] - id0] = r
         if time.perf_counter() > deadline:
             raise RuntimeError(f"stream did not finish in {timeout} s ({len(results)}/{len(reqs)})")
         time.sleep(0.0005)
     wall = time.perf_counter() - t0
-    out_fields = {i: srv.field(i) for i in results} if fields else None
+    out_fields = {i: srv.field(id0 + i) for i in results} if fields else None
     stats = srv.stats()
-    srv.close()
+    if server is None:
+        srv.close()
     return wall, results, stats, out_fields
 
 
@@ -380,11 +383,18 @@ def main():
     # and every request pays its set-up inside the timed region.
     R = a.requests
 
-    # warmup: short streams of requests outside the timed seeds (plans,
-    # twiddles, allocator, code objects)
+    # One long-lived server takes the warm-up and the timed steps, as a
+    # service would; every step still submits R fresh requests and waits for
+    # all of them (ramp-up and tail included).
+    server = native.Server(precision="fp32_refined", max_slots=a.slots, chunk=a.chunk, keep_fields=False,
+                           precond=a.precond)
+    next_id = 0
+    # warmup: requests outside the timed seeds (plans, twiddles, allocator,
+    # system pool, code objects)
     for w in range(a.warmup):
-        run_stream(native, make_requests(native, min(48, R), seed0=900000 + 48 * w), "fp32_refined", a.slots,
-                   a.chunk, precond=a.precond)
+        wreqs = make_requests(native, min(128, R), seed0=900000 + 128 * w)
+        run_stream(native, wreqs, "fp32_refined", a.slots, a.chunk, server=server, id0=next_id)
+        next_id += len(wreqs)
     print("# warmup done", file=sys.stderr, flush=True)
 
     # timed steps: saturation throughput of the refined batched server
@@ -394,7 +404,9 @@ def main():
         reqs = make_requests(native, R, seed0=1000 + R * k)
         if k == 0:
             reqs0 = reqs
-        w, res, stats_sat, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk, precond=a.precond)
+        w, res, stats_sat, _ = run_stream(native, reqs, "fp32_refined", a.slots, a.chunk, server=server,
+                                          id0=next_id)
+        next_id += R
         walls.append(w)
         res_steps.append(res)
         hits += sum(1 for r in res.values() if r["operator_cached"])
@@ -404,6 +416,7 @@ def main():
     res_all = [r for res in res_steps for r in res.values()]
     conv = sum(1 for r in res_all if r["status"] == "converged")
     if a.saturation_only:
+        server.close()
         print(json.dumps({"saturation_solves_per_s": round(value, 3), "converged": conv, "operator_cache_hits": hits,
                           "server": stats_sat}), flush=True)
         return
@@ -415,7 +428,9 @@ def main():
     arrivals -= arrivals[0]
     reqs_p = make_requests(native, R, seed0=1000 + R * a.steps)
     w_p, res_p, stats_p, _ = run_stream(native, reqs_p, "fp32_refined", a.slots, a.chunk, arrivals=arrivals,
-                                        precond=a.precond)
+                                        server=server, id0=next_id)
+    next_id += R
+    server.close()
     lat = [(r["t_done"] - r["t_submit"]) * 1e3 for r in res_p.values()]
     print(f"# poisson stream done: p50 {percentile(lat, 0.5):.2f} ms", file=sys.stderr, flush=True)
 

@@ -368,7 +368,7 @@ int tdda_problem_grid(tdda_problem* p, int* lxyz, double* setup_ms, int* cached)
             lxyz[1] = op.ly;
             lxyz[2] = op.lz;
         }
-        if (setup_ms) *setup_ms = op.setup_ms;
+        if (setup_ms) *setup_ms = op.setup_ms();
         if (cached) *cached = p->p->op_hit ? 1 : 0;
     });
 }

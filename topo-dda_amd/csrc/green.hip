@@ -80,35 +80,59 @@ __global__ void __launch_bounds__(256) k_green_lattice(GreenParams P, const doub
     }
 }
 
-// out[c][a][k][i] = sum_o W_c(k,o) in[c][a][o][i], 0 <= k < h, 0 <= o < no
+// out[c][a][k][i] = sum_o W_c(k,o) in[c][a][o][i], 0 <= k < h, 0 <= o < no.
+// A thread computes GREEN_KB outputs k = kb + j nk of one column (j < KB,
+// nk = ceil(h / KB)), so every input it loads feeds KB products; the twiddle
+// table sits in LDS.  Consecutive threads take consecutive columns i (then
+// consecutive kb), so the input loads are coalesced along the contiguous axis
+// or shared by a wave when inner = 1.
+constexpr int GREEN_KB = 4;
+constexpr int GREEN_MAX_L = 256;
 __global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
                                                    int outer, int no, int h, int inner, int L, int axis,
                                                    const double2* __restrict__ tw) {
-    const long total = 6L * outer * h * inner;
+    __shared__ double2 twl[GREEN_MAX_L];
+    for (int t = threadIdx.x; t < L; t += blockDim.x) twl[t] = tw[t];
+    __syncthreads();
+    const int nk = (h + GREEN_KB - 1) / GREEN_KB;
+    const long total = 6L * outer * nk * inner;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int i = idx % inner;
-    const int k = (idx / inner) % h;
-    const long a = (idx / ((long)inner * h)) % outer;
-    const int c = idx / ((long)inner * h * outer);
+    const int i = (int)(idx % inner);
+    const int kb = (int)((idx / inner) % nk);
+    const long a = (idx / ((long)inner * nk)) % outer;
+    const int c = (int)(idx / ((long)inner * nk * outer));
     const bool odd = c_odd[c][axis] != 0;
     const double2* src = in + ((long)c * outer + a) * no * inner + i;
-    double2 acc = make_double2(0.0, 0.0);
+    int k[GREEN_KB], m[GREEN_KB];
+    double2 acc[GREEN_KB];
+#pragma unroll
+    for (int j = 0; j < GREEN_KB; ++j) {
+        k[j] = min(kb + j * nk, h - 1);   // clamped lanes compute a duplicate that is not stored
+        m[j] = 0;                         // phase index k o mod L, advanced by k per input
+        acc[j] = make_double2(0.0, 0.0);
+    }
     for (int o = 0; o < no; ++o) {
         const double2 v = src[(long)o * inner];
-        const int m = (int)(((long)k * o) % L);
-        const double cs = tw[m].x, sn = -tw[m].y;   // tw[m] = exp(-2 pi i m / L)
-        if (!odd) {
-            const double w = o == 0 ? 1.0 : 2.0 * cs;
-            acc.x += w * v.x;
-            acc.y += w * v.y;
-        } else if (o > 0) {
-            const double s2 = 2.0 * sn;   // (-i s2) v
-            acc.x += s2 * v.y;
-            acc.y -= s2 * v.x;
+#pragma unroll
+        for (int j = 0; j < GREEN_KB; ++j) {
+            const double2 w = twl[m[j]];   // exp(-2 pi i m / L)
+            if (!odd) {
+                const double f = o == 0 ? 1.0 : 2.0 * w.x;   // 2 cos
+                acc[j].x += f * v.x;
+                acc[j].y += f * v.y;
+            } else if (o > 0) {
+                const double s2 = -2.0 * w.y;   // 2 sin; (-i s2) v
+                acc[j].x += s2 * v.y;
+                acc[j].y -= s2 * v.x;
+            }
+            m[j] = (m[j] + k[j] >= L) ? m[j] + k[j] - L : m[j] + k[j];
         }
     }
-    out[(((long)c * outer + a) * h + k) * inner + i] = acc;
+    double2* dst = out + ((long)c * outer + a) * h * inner + i;
+#pragma unroll
+    for (int j = 0; j < GREEN_KB; ++j)
+        if (kb + j * nk < h) dst[(long)(kb + j * nk) * inner] = acc[j];
 }
 
 __global__ void k_green_interleave(const double2* __restrict__ in, long rows, int hx, double2* __restrict__ out64,
@@ -135,7 +159,8 @@ void launch_green_octant(const GreenParams& P, const double* si, const double* c
 
 void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
                       const double2* tw, hipStream_t st) {
-    const long total = 6L * outer * h * inner;
+    TDDA_REQUIRE(L <= GREEN_MAX_L, ERR_INVALID, "transform length beyond the Green's DFT table");
+    const long total = 6L * outer * ((h + GREEN_KB - 1) / GREEN_KB) * inner;
     hipLaunchKernelGGL(k_green_dft, dim3((total + 255) / 256), dim3(256), 0, st, in, out, outer, no, h, inner, L,
                        axis, tw);
     TDDA_HIP(hipGetLastError());

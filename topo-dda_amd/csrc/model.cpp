@@ -139,7 +139,7 @@ Model::Model(const ModelConfig& cfg) : cfg_(cfg) {
     OperatorKey key{cfg.nx, cfg.ny, cfg.nz, cfg.d, K_, method, cfg.maxm, cfg.maxn,
                     cfg.lm * cfg.d, cfg.ln * cfg.d, method == 0 ? tables.id : 0};
     op_ = OperatorCache::instance().get(key, tables, default_stream(), &op_hit_);
-    op_setup_ms_ = op_->setup_ms;
+    op_setup_ms_ = op_->setup_ms();
 
     // objective (ObjDDAModel.cpp:103-128), incident field (DDAModel.cpp:312-316)
     pownum_ = (int)c_round(cfg.objpara[0]);

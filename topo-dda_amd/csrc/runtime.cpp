@@ -215,7 +215,6 @@ bool OperatorKey::operator<(const OperatorKey& o) const {
 }
 
 Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t st) : key(k) {
-    auto t0 = std::chrono::steady_clock::now();
     nx = k.nx; ny = k.ny; nz = k.nz;
     TDDA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, ERR_INVALID, "grid dimensions must be positive");
     lx = choose_fft_length(nx);
@@ -238,20 +237,47 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     P.maxm = k.maxm; P.maxn = k.maxn; P.lm = k.lm; P.ln = k.ln;
     P.nsici = tables.n; P.delta = tables.delta;
 
+    TDDA_HIP(hipEventCreate(&start_));
+    TDDA_HIP(hipEventCreate(&done_));
     const long noct = (long)nx * ny * nz;
-    DevBuf<double2> a(6 * noct);
-    DevBuf<double2> b(6L * nz * ny * hx);
-    DevBuf<double2> c(6L * nz * hy * hx);
+    tmp_.resize(4);
+    tmp_[0].alloc(6 * noct);
+    tmp_[1].alloc(6L * nz * ny * hx);
+    tmp_[2].alloc(6L * nz * hy * hx);
+    tmp_[3].alloc(6L * hz * hy * hx);
     green64.alloc(6L * hz * hy * hx);
     green32.alloc(6L * hz * hy * hx);
-    launch_green_octant(P, tables.si, tables.ci, a.get(), st);
-    launch_green_dft(a.get(), b.get(), nz * ny, nx, hx, 1, lx, 0, tw.tw64 + px.tw_off, st);
-    launch_green_dft(b.get(), c.get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
-    DevBuf<double2> e(6L * hz * hy * hx);
-    launch_green_dft(c.get(), e.get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
-    launch_green_interleave(e.get(), (long)hz * hy, hx, green64.get(), green32.get(), st);
-    TDDA_HIP(hipStreamSynchronize(st));
-    setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    TDDA_HIP(hipEventRecord(start_, st));
+    launch_green_octant(P, tables.si, tables.ci, tmp_[0].get(), st);
+    launch_green_dft(tmp_[0].get(), tmp_[1].get(), nz * ny, nx, hx, 1, lx, 0, tw.tw64 + px.tw_off, st);
+    launch_green_dft(tmp_[1].get(), tmp_[2].get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
+    launch_green_dft(tmp_[2].get(), tmp_[3].get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
+    launch_green_interleave(tmp_[3].get(), (long)hz * hy, hx, green64.get(), green32.get(), st);
+    TDDA_HIP(hipEventRecord(done_, st));
+}
+
+Operator::~Operator() {
+    // the temporaries and the spectrum may still be in use by the build
+    if (done_) (void)hipEventSynchronize(done_);
+    tmp_.clear();
+    if (start_) (void)hipEventDestroy(start_);
+    if (done_) (void)hipEventDestroy(done_);
+}
+
+double Operator::setup_ms() const {
+    TDDA_HIP(hipEventSynchronize(done_));
+    float ms = 0.f;
+    TDDA_HIP(hipEventElapsedTime(&ms, start_, done_));
+    return ms;
+}
+
+bool Operator::reap() {
+    if (tmp_.empty()) return true;
+    const hipError_t e = hipEventQuery(done_);
+    if (e == hipErrorNotReady) return false;
+    TDDA_HIP(e);
+    tmp_.clear();
+    return true;
 }
 
 OperatorCache& OperatorCache::instance() {
@@ -288,6 +314,15 @@ void OperatorCache::clear() {
     lru_.clear();
 }
 
+void OperatorCache::reap_locked() {
+    size_t k = 0;
+    for (size_t i = 0; i < unreaped_.size(); ++i) {
+        std::shared_ptr<Operator> op = unreaped_[i].lock();
+        if (op && !op->reap()) unreaped_[k++] = unreaped_[i];
+    }
+    unreaped_.resize(k);
+}
+
 void OperatorCache::touch_locked(const OperatorKey& key) {
     auto pos = std::find_if(lru_.begin(), lru_.end(),
                             [&](const OperatorKey& k) { return !(k < key) && !(key < k); });
@@ -302,10 +337,13 @@ std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciD
     {
         std::unique_lock<std::mutex> lk(mu_);
         built_.wait(lk, [&] { return building_.count(key) == 0; });
+        reap_locked();
         auto it = map_.find(key);
         if (it != map_.end()) {
             touch_locked(key);
             if (hit) *hit = true;
+            // a build enqueued on another stream may still run
+            TDDA_HIP(hipStreamWaitEvent(st, it->second->ready(), 0));
             return it->second;
         }
         building_.insert(key);
@@ -326,6 +364,7 @@ std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciD
         std::lock_guard<std::mutex> lk(mu_);
         building_.erase(key);
         map_[key] = op;
+        unreaped_.push_back(op);
         touch_locked(key);
         size_t total = 0;
         for (auto& kv : map_) total += kv.second->bytes();
@@ -485,6 +524,33 @@ void System::get_solution(double2* h, hipStream_t st) const { P.download(h, 3ull
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+PinnedStage::~PinnedStage() {
+    if (p_) (void)hipHostFree(p_);
+}
+
+char* PinnedStage::reserve(size_t bytes) {
+    if (bytes > cap_) {
+        if (p_) TDDA_HIP(hipHostFree(p_));
+        p_ = nullptr;
+        const size_t c = std::max(bytes, 2 * cap_);
+        TDDA_HIP(hipHostMalloc((void**)&p_, c, hipHostMallocDefault));
+        cap_ = c;
+    }
+    return p_;
+}
+
+template <typename T>
+void Batch::upload(DevBuf<T>& dst, const T* src, size_t n, size_t stage_off, hipStream_t st) {
+    if (n == 0) return;
+    if (!pinned_) {
+        dst.upload(src, n, st);
+        return;
+    }
+    char* stage = stage_.reserve(0) + stage_off;
+    std::memcpy(stage, src, n * sizeof(T));
+    dst.upload(reinterpret_cast<const T*>(stage), n, st);
+}
+
 void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st) {
     std::vector<KScalars*> ks(systems.size(), nullptr);
     if (ks_dev)
@@ -633,8 +699,15 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             }
     const size_t off_u = all.size();
     all.insert(all.end(), mu.begin(), mu.end());
+    // stage layout: jobs, maps, reliable-update ranges (256-byte aligned)
+    const int nj_all = (int)hjobs.size();
+    auto al256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    stage_jobs_ = 0;
+    const size_t stage_maps = al256(hjobs.size() * sizeof(GridJob));
+    const size_t stage_ranges = stage_maps + al256(total * sizeof(int2));
+    if (pinned_) stage_.reserve(stage_ranges + al256((size_t)RU_REGIONS * (nj_all + 1) * sizeof(int)));
     if (maps_.size() < total) maps_.alloc(std::max<size_t>(total, 1));
-    if (total) maps_.upload(all.data(), total, st);
+    upload(maps_, all.data(), total, stage_maps, st);
     if (djobs_.size() < hjobs.size()) djobs_.alloc(std::max<size_t>(hjobs.size(), 1));
     upload_jobs(st);
     const int2* base = maps_.get();
@@ -685,14 +758,12 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
                 }
             }
         if (ru_ranges_.size() < rr.size()) ru_ranges_.alloc(rr.size());
-        ru_ranges_.upload(rr.data(), rr.size(), st);
+        upload(ru_ranges_, rr.data(), rr.size(), stage_ranges, st);
         ls.ru_ranges = ru_ranges_.get();
     }
 }
 
-void Batch::upload_jobs(hipStream_t st) {
-    if (!hjobs.empty()) djobs_.upload(hjobs.data(), hjobs.size(), st);
-}
+void Batch::upload_jobs(hipStream_t st) { upload(djobs_, hjobs.data(), hjobs.size(), stage_jobs_, st); }
 
 // ---------------------------------------------------------------------------
 // solving

@@ -114,16 +114,30 @@ struct OperatorKey {
     bool operator<(const OperatorKey& o) const;
 };
 
+// The build is enqueued on the caller's stream and not waited for: work on
+// that stream sees the spectrum; other streams wait on ready() (the cache
+// does this on every hit).  The build's temporaries live until it is done.
 class Operator {
    public:
     Operator(const OperatorKey& key, const SiciDevice& tables, hipStream_t st);
+    ~Operator();
+    Operator(const Operator&) = delete;
+    Operator& operator=(const Operator&) = delete;
     OperatorKey key;
     int nx, ny, nz, lx, ly, lz, hx, hy, hz;
     FftPlan px, py, pz;
     DevBuf<double2> green64;   // [hz][hy][6][hx] (c64: [hz][hy][3][hx] of component pairs)
     DevBuf<float2> green32;
-    double setup_ms = 0.0;
     size_t bytes() const { return green64.bytes() + green32.bytes(); }
+    hipEvent_t ready() const { return done_; }
+    // GPU time of the build (lattice sum, DFTs, layout) in ms; waits for it
+    double setup_ms() const;
+    // free the build temporaries once the build has run; true when none are left
+    bool reap();
+
+   private:
+    hipEvent_t start_ = nullptr, done_ = nullptr;
+    std::vector<DevBuf<double2>> tmp_;
 };
 
 // LRU cache of operators bounded by bytes
@@ -140,10 +154,12 @@ class OperatorCache {
 
    private:
     void touch_locked(const OperatorKey& key);   // move key to the most recent end of lru_
+    void reap_locked();                          // free finished builds' temporaries
     mutable std::mutex mu_;
     std::condition_variable built_;
     std::map<OperatorKey, std::shared_ptr<Operator>> map_;
     std::set<OperatorKey> building_;   // keys one thread is building (others wait for it)
+    std::vector<std::weak_ptr<Operator>> unreaped_;   // builds whose temporaries are still held
     std::vector<OperatorKey> lru_;     // each cached key once, least recent first
     size_t budget_ = size_t(8) << 30;
 };
@@ -209,9 +225,28 @@ struct SolveResult {
     double err_est = 0.0;   // refined mode: estimated relative field error (0: none)
 };
 
+// Grow-only pinned host buffer: asynchronous uploads from it do not wait for
+// the stream (uploads from pageable memory block the host until the stream
+// reaches them).  The owner must not overwrite bytes an upload still reads.
+class PinnedStage {
+   public:
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage();
+    char* reserve(size_t bytes);   // contents are not kept on growth
+   private:
+    char* p_ = nullptr;
+    size_t cap_ = 0;
+};
+
 // Device-resident batch: descriptors and (job, tile) maps for every pass.
 class Batch {
    public:
+    // Upload through a pinned stage (batches rebuilt while the GPU runs:
+    // the stage of a build must not be rebuilt before its uploads ran, which
+    // the server guarantees by alternating two batches behind events).
+    void set_pinned(bool on) { pinned_ = on; }
     // ks_dev: scalars of system j at ks_dev + j (or null for plain operator use)
     void build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st);
     // explicit scalar slot per system; systems that share an Operator get
@@ -224,11 +259,15 @@ class Batch {
     std::vector<GridJob> hjobs;
 
    private:
+    template <typename T> void upload(DevBuf<T>& dst, const T* src, size_t n, size_t stage_off, hipStream_t st);
     DevBuf<GridJob> djobs_;
     DevBuf<int2> maps_, ru_maps_;
     DevBuf<int> ru_count_, ru_ranges_;
     std::vector<int> hranges_;
     std::vector<int2> hmaps_;
+    bool pinned_ = false;
+    PinnedStage stage_;
+    size_t stage_jobs_ = 0;   // offsets of the regions in the stage
 };
 
 // Solve every system in the batch with BiCGSTAB (DDAModel.cpp:899-1009).

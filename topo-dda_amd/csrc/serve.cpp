@@ -14,11 +14,16 @@ Server::Server(const ServerOptions& o, const SiciDevice& tables) : opt_(o), tabl
     TDDA_HIP(hipStreamCreate(&setup_));
     ks_.alloc(o.max_slots);
     ks_.zero(main_);
+    for (int i = 0; i < 2; ++i) {   // rebuilt while the GPU runs the other batch
+        batch_[i].set_pinned(true);
+        init_batch_[i].set_pinned(true);
+    }
     for (int i = 0; i < 2; ++i) {
         TDDA_HIP(hipHostMalloc((void**)&snap_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
         TDDA_HIP(hipHostMalloc((void**)&init_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
         std::memset(snap_[i], 0, o.max_slots * sizeof(KScalars));
         TDDA_HIP(hipEventCreateWithFlags(&chunk_done_[i], hipEventDisableTiming));
+        TDDA_HIP(hipEventCreateWithFlags(&rho_done_[i], hipEventDisableTiming));
     }
     TDDA_HIP(hipStreamSynchronize(main_));
     slots_.resize(o.max_slots);
@@ -40,6 +45,7 @@ Server::~Server() {
         if (snap_[i]) (void)hipHostFree(snap_[i]);
         if (init_[i]) (void)hipHostFree(init_[i]);
         if (chunk_done_[i]) (void)hipEventDestroy(chunk_done_[i]);
+        if (rho_done_[i]) (void)hipEventDestroy(rho_done_[i]);
     }
     (void)hipStreamDestroy(main_);
     (void)hipStreamDestroy(setup_);
@@ -104,6 +110,7 @@ std::vector<int> Server::admit() {
     std::vector<int> joined;
     while (true) {
         ServeRequest req;
+        if ((int)joined.size() >= opt_.admit_per_round) break;
         {
             std::lock_guard<std::mutex> lk(mu_);
             if (pending_.empty()) break;
@@ -134,15 +141,31 @@ std::vector<int> Server::admit() {
             pit->second.pop_back();
         }
         sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
-        sl.prob->set_density(req.rho.data(), false, setup_);
         sl.prob->sys->P.zero(setup_);
         sl.prob->sys->set_jacobi(opt_.jacobi);
         sl.op_hit = sl.prob->op_hit ? 1 : 0;
         sl.t_admit = now();
         sl.req = std::move(req);
+    }
+    if (joined.empty()) return joined;
+    // densities through this round's pinned stage (its previous round has
+    // uploaded once its event completed)
+    const int r = rho_round_;
+    rho_round_ ^= 1;
+    if (rho_pending_[r]) TDDA_HIP(hipEventSynchronize(rho_done_[r]));
+    size_t total = 0;
+    for (int s : joined) total += slots_[s].req.rho.size();
+    double* stage = reinterpret_cast<double*>(rho_stage_[r].reserve(total * sizeof(double)));
+    for (int s : joined) {
+        Slot& sl = slots_[s];
+        std::memcpy(stage, sl.req.rho.data(), sl.req.rho.size() * sizeof(double));
+        sl.prob->set_density(stage, false, setup_);
+        stage += sl.req.rho.size();
         sl.req.rho.clear();
         sl.req.rho.shrink_to_fit();
     }
+    TDDA_HIP(hipEventRecord(rho_done_[r], setup_));
+    rho_pending_[r] = true;
     return joined;
 }
 

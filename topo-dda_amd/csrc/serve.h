@@ -31,6 +31,7 @@ struct ServerOptions {
     size_t hbm_budget = size_t(200) << 30;
     int max_slots = 64;
     int chunk = 4;                   // iterations between host checks
+    int admit_per_round = 32;        // admissions between two chunks (the first chunk starts early)
     bool keep_fields = true;         // keep E for take_field()
 };
 
@@ -109,6 +110,12 @@ class Server {
     std::vector<Slot> slots_;
     std::vector<std::pair<int, unsigned>> members_[2];   // (slot, generation) in batch i
     Batch batch_[2], init_batch_[2];
+    // densities of an admission round go up through a pinned stage; two
+    // stages alternate, each reused after the round two back has uploaded
+    PinnedStage rho_stage_[2];
+    hipEvent_t rho_done_[2] = {nullptr, nullptr};
+    bool rho_pending_[2] = {false, false};
+    int rho_round_ = 0;
     std::map<std::tuple<int, int, int>, std::vector<std::unique_ptr<System>>> pool_;
     int busy_ = 0;   // requests admitted but not finished (guarded by mu_)
 };
