@@ -206,6 +206,16 @@ int tdda_problem_gradient(tdda_problem* p, int kind, int power, const double* ta
  * precision TDDA_PREC_FP32_REFINED) or FP64 ones.  x, y: host arrays of the
  * problems' vectors back to back, 3*nx*ny*nz complex each. */
 int tdda_problems_apply(tdda_problem** ps, int n, int c64, const double* x, double* y);
+/* y = (diag(al) + A) x with FP64 transforms for one problem, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  x, y: DEVICE arrays of 3*nx*ny*nz
+ * complex (interleaved doubles) that the caller owns.  The job descriptor and
+ * pass maps go to a fixed 256 KiB device array per GPU (a problem needs 1 KiB
+ * plus 8 bytes per pass tile; larger ones are refused with TDDA_ERR_MEMORY).
+ * The work is enqueued on the library stream and not waited for; calls that
+ * overlap on one GPU share that array and race. */
+int tdda_problem_apply_noalloc(tdda_problem* p, const double* x, double* y);
 /* Average ms per launch of the five matvec passes over all n problems in one
  * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
  * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */

@@ -4,6 +4,7 @@
 
 #include "../../include/topodda.h"
 #include "model.h"
+#include "noalloc.h"
 #include "objective.h"
 #include "serve.h"
 
@@ -447,6 +448,14 @@ int tdda_problems_apply(tdda_problem** ps, int n, int c64, const double* x, doub
         }
         tdda::apply_batch(sys, c64 != 0, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y),
                           tdda::default_stream());
+    });
+}
+
+int tdda_problem_apply_noalloc(tdda_problem* p, const double* x, double* y) {
+    return guarded([&] {
+        TDDA_REQUIRE(p && p->p, tdda::ERR_INVALID, "null problem");
+        tdda::apply_noalloc(*p->p->sys, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y),
+                            tdda::default_stream());
     });
 }
 
