@@ -188,7 +188,7 @@ int main() {
     for (int n : {32, 31, 34, 64, 1}) { check_z<16, 4, float>(n); check_z<16, 4, double>(n); }
     printf("Z splits with L2 = 4 done (worst so far %.3e)\n", worst);
     // X / Y splits with L1 = 8
-    check_len<8, 10>(); check_len<8, 12>(); check_len<8, 16>();
+    check_len<8, 10>(); check_len<8, 12>(); check_len<8, 16>(); check_len<16, 16>();
     printf("worst rel err %.3e\n", worst);
     return worst > 1e-5 ? 1 : 0;
 }

@@ -524,6 +524,7 @@ TDDA_HD bool with_split(int L, F&& f) {
         case 80: f(IC<10>{}, IC<8>{}); return true;
         case 96: f(IC<12>{}, IC<8>{}); return true;
         case 128: f(IC<16>{}, IC<8>{}); return true;
+        case 256: f(IC<16>{}, IC<16>{}); return true;
         default: return false;
     }
 }
@@ -550,6 +551,9 @@ TDDA_HD bool with_split_z(int L, F&& f) {
         case 40: f(IC<10>{}, IC<4>{}); return true;
         case 48: f(IC<12>{}, IC<4>{}); return true;
         case 64: f(IC<16>{}, IC<4>{}); return true;
+        // 256 keeps the generic Z pass: a 16 x 16 tile of three components
+        // needs about 100 KB of LDS per block
+        case 256: return false;
         default: return with_split(L, f);
     }
 }
