@@ -140,6 +140,10 @@ int tdda_model_solve_resident(tdda_model* m, int maxit, double tol, tdda_solve_i
  * the matvec is AProductCore::Aproduct + al*b (src/AProductCore.cpp:461-515). */
 int tdda_model_time_passes(tdda_model* m, int reps, double* ms5);
 
+/* Batched solves record one BiCGSTAB iteration as a HIP graph and replay it
+ * (on = 1, the default); 0 issues the same launches one by one.  Results are
+ * the same either way.  No reference counterpart. */
+int tdda_set_graph_capture(int on);
 /* Bytes of device memory currently held by the library. */
 long tdda_device_bytes(void);
 /* Drop cached Green's spectra nobody uses and return cached device blocks to
@@ -230,7 +234,7 @@ typedef struct {
     int precision;       /* TDDA_PREC_* */
     int maxit;
     double tol;
-    double ru_delta;     /* reliable-update trigger (refined precision) */
+    double ru_delta;     /* reliable-update trigger (refined precision); 0 = 0.01 */
     double hbm_budget_gb;
     int max_slots;       /* requests in flight */
     int chunk;           /* iterations between host checks */

@@ -220,3 +220,45 @@ def test_server_jacobi_mix_equals_alone_and_oracle(native, tables):
             err = _rel(mix[i], _oracle_field(O, r, tables))
             _log(f"[gpu] jacobi served {i}: {got[i]['iterations']} its, E err {err:.3e}, estimate {got[i]['err_est']:.3e}")
             assert err <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_graph_replay_equals_eager(native, tables, precision):
+    """Batched solves replay one recorded BiCGSTAB iteration per step (a HIP
+    graph, launched hundreds of times with the host's chunk checks in
+    between); the same solves with the launches issued one by one give the
+    same iterations and bitwise the same fields, every replay having fed the
+    next.  The server's chunk graphs (recorded once a batch is reused, as in
+    the tail of a stream) are checked the same way."""
+    _log(f"[gpu] graph vs eager {precision}")
+    reqs = [_request(native, s, seed=500 + i, lam=470 + 55 * i, theta=0.15 * i, psi=0.4 * i)
+            for i, s in enumerate([(16, 16, 16), (20, 12, 10), (24, 24, 24)])]
+    out = {}
+    for on in (True, False):
+        native.set_graph_capture(on)
+        try:
+            for pc in ("none", "jacobi"):
+                ps = [native.Problem(r, precision=precision, tables=tables, precond=pc) for r in reqs]
+                infos = native.solve(ps)
+                out[(on, pc)] = ([i["iterations"] for i in infos], [p.field() for p in ps])
+            srv = native.Server(precision=precision, max_slots=4, chunk=2, precond="jacobi")
+            for i, r in enumerate(reqs):
+                srv.submit(i, r)
+            got = {}
+            import time
+            t_end = time.time() + 60
+            while len(got) < len(reqs) and time.time() < t_end:
+                for res in srv.poll():
+                    got[res["id"]] = res
+                time.sleep(0.002)
+            out[(on, "server")] = ([got[i]["iterations"] for i in range(len(reqs))],
+                                   [srv.field(i) for i in range(len(reqs))])
+            srv.close()
+        finally:
+            native.set_graph_capture(True)
+    for key in ("none", "jacobi", "server"):
+        its_g, f_g = out[(True, key)]
+        its_e, f_e = out[(False, key)]
+        assert its_g == its_e, key
+        for a, b in zip(f_g, f_e):
+            assert np.array_equal(a, b), key

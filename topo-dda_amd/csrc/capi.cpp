@@ -46,6 +46,10 @@ const char* tdda_last_error(void) { return tdda::get_last_error(); }
 
 long tdda_device_bytes(void) { return (long)tdda::device_bytes_in_use(); }
 
+int tdda_set_graph_capture(int on) {
+    return guarded([&] { tdda::set_graph_capture(on != 0); });
+}
+
 int tdda_trim_caches(void) {
     return guarded([&] {
         tdda::OperatorCache::instance().trim(~size_t(0));
@@ -491,7 +495,7 @@ int tdda_server_create(const tdda_server_opts* o, const double* si, const double
         so.refined = o->precision == TDDA_PREC_FP32_REFINED;
         so.maxit = o->maxit;
         so.tol = o->tol;
-        so.ru_delta = o->ru_delta > 0 ? o->ru_delta : 0.1;
+        so.ru_delta = o->ru_delta > 0 ? o->ru_delta : 0.01;
         so.err_tol = o->err_tol > 0 ? o->err_tol : (o->err_tol == 0 ? o->tol : 0.0);
         so.hbm_budget = (size_t)(o->hbm_budget_gb * (double)(1ull << 30));
         so.max_slots = o->max_slots;

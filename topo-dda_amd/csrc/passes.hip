@@ -356,6 +356,10 @@ k_ru_compact(const GridJob* __restrict__ jobs, int njobs, const int2* __restrict
     }
 }
 
+__global__ void k_clear_ints(int* __restrict__ p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // ----------------------------------------------------------------------------
 // per-job sums of the X-inverse dot partials and the scalar step (one block
 // per job, after the X inverse launches of a matvec)
@@ -530,8 +534,10 @@ static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int
     const int pr = sizeof(Real) == 4 ? 0 : 1;
     if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
     if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_ranges != nullptr && ls.ru_off[RU_REGIONS] > 0) {
-        // only the tiles of jobs that wait for their reliable update
-        TDDA_HIP(hipMemsetAsync(ls.ru_count, 0, RU_REGIONS * sizeof(int), st));
+        // only the tiles of jobs that wait for their reliable update (the
+        // counts are cleared by a kernel, not a memset: this sequence is
+        // captured into graphs, where memset nodes are avoided)
+        hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, st, ls.ru_count, RU_REGIONS);
         RuRegions rg;
         for (int k = 0; k <= RU_REGIONS; ++k) rg.off[k] = ls.ru_off[k];
         hipLaunchKernelGGL(k_ru_compact, dim3(ls.njobs), dim3(FFT_THREADS), 0, st, ls.jobs, ls.njobs, ls.ru_src, rg,

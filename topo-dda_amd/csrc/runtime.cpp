@@ -766,6 +766,50 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
 void Batch::upload_jobs(hipStream_t st) { upload(djobs_, hjobs.data(), hjobs.size(), stage_jobs_, st); }
 
 // ---------------------------------------------------------------------------
+// graphs
+// ---------------------------------------------------------------------------
+namespace {
+std::atomic<bool> g_graphs{true};
+
+struct CaptureStream {
+    hipStream_t s = nullptr;
+    ~CaptureStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+}  // namespace
+
+void set_graph_capture(bool on) { g_graphs = on; }
+bool graph_capture() { return g_graphs; }
+
+hipGraphExec_t capture_graph(const std::function<void(hipStream_t)>& enqueue) {
+    static thread_local CaptureStream cs;
+    if (!cs.s) TDDA_HIP(hipStreamCreateWithFlags(&cs.s, hipStreamNonBlocking));
+    TDDA_HIP(hipStreamBeginCapture(cs.s, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue(cs.s);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cs.s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    TDDA_HIP(hipStreamEndCapture(cs.s, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TDDA_HIP(rc);
+    return e;
+}
+
+void launch_iteration(const LaunchSet& ls, const TwiddlePools& tw, bool refined, hipStream_t st) {
+    launch_matvec(ls, tw, refined, XF_P, XI_V, GATE_ACTIVE, st);
+    launch_matvec(ls, tw, refined, XF_T, XI_S, GATE_ACTIVE, st);
+    if (refined) launch_matvec(ls, tw, false, XF_SOL, XI_RESID, GATE_RU, st);
+}
+
+// ---------------------------------------------------------------------------
 // solving
 // ---------------------------------------------------------------------------
 std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const SolveOptions& opt, hipStream_t st) {
@@ -804,14 +848,21 @@ std::vector<SolveResult> solve_batch(const std::vector<System*>& systems, const 
     if (opt.warm)
         for (System* s : systems)
             if (s->jacobi()) s->scale_solution(true, st);
-    const bool c64 = opt.refined;
     std::vector<KScalars> snap(nj);
     int chunk = opt.chunk > 0 ? opt.chunk : 4;
+    // one iteration recorded once, replayed for every iteration
+    struct ExecGuard {
+        hipGraphExec_t e = nullptr;
+        ~ExecGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } iter_graph;
+    if (graph_capture())
+        iter_graph.e = capture_graph([&](hipStream_t cs) { launch_iteration(b.ls, tw, opt.refined, cs); });
     while (true) {
         for (int k = 0; k < chunk; ++k) {
-            launch_matvec(b.ls, tw, c64, XF_P, XI_V, GATE_ACTIVE, st);
-            launch_matvec(b.ls, tw, c64, XF_T, XI_S, GATE_ACTIVE, st);
-            if (opt.refined) launch_matvec(b.ls, tw, false, XF_SOL, XI_RESID, GATE_RU, st);
+            if (iter_graph.e) TDDA_HIP(hipGraphLaunch(iter_graph.e, st));
+            else launch_iteration(b.ls, tw, opt.refined, st);
         }
         launch_finalize(b.ls, st);
         ks.download(snap.data(), nj, st);

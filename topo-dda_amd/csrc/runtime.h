@@ -3,6 +3,7 @@
 #pragma once
 
 #include <condition_variable>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -208,13 +209,24 @@ struct SolveOptions {
     bool refined = false;   // c64 Krylov + FP64 reliable updates (requires refined systems)
     int maxit = 100000;
     double tol = 1e-5;
-    double ru_delta = 0.1;
+    double ru_delta = 0.01;   // reliable update when the residual falls this far below its peak since the last one
     // refined mode: relative field-error target (estimated from the solve
     // itself, see KScalars); 0: stop on the true residual alone
     double err_tol = 0.0;
     bool warm = false;      // use the system's current P as the initial guess
     int chunk = 0;          // iterations between host checks (0: adaptive)
 };
+
+// Graph capture of launch sequences (on by default): the sequence is
+// recorded once on a thread-local side stream and replayed with one graph
+// launch.  Off: the same launches are issued one by one.
+void set_graph_capture(bool on);
+bool graph_capture();
+// Records enqueue(stream) into an executable graph (thread-local capture mode).
+hipGraphExec_t capture_graph(const std::function<void(hipStream_t)>& enqueue);
+// One BiCGSTAB step of a batch as a graph: the p and t matvecs and, in refined
+// mode, the reliable-update matvec (gated per job on the device).
+void launch_iteration(const LaunchSet& ls, const TwiddlePools& tw, bool refined, hipStream_t st);
 
 struct SolveResult {
     int status = JOB_IDLE;

@@ -39,6 +39,15 @@ Server::~Server() {
     if (th_.joinable()) th_.join();
     (void)hipStreamSynchronize(main_);
     (void)hipStreamSynchronize(setup_);
+    drop_graph(0);
+    drop_graph(1);
+    for (auto& rt : retiring_) {
+        if (rt.ev) (void)hipEventDestroy(rt.ev);
+        if (rt.host) (void)hipHostFree(rt.host);
+    }
+    retiring_.clear();
+    for (auto& kv : pinned_fields_)
+        for (double2* h : kv.second) (void)hipHostFree(h);
     slots_.clear();
     pool_.clear();
     for (int i = 0; i < 2; ++i) {
@@ -169,49 +178,113 @@ std::vector<int> Server::admit() {
     return joined;
 }
 
+// Finished requests leave their slot at once.  Without kept fields they are
+// done; with them, the field is formed and copied to pinned memory on the
+// compute stream behind an event, and the request completes when that copy
+// has landed (complete_retirements), so the loop never waits for the stream.
 void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
     if (fin.empty()) return;
-    std::vector<DevBuf<double2>> ebuf(fin.size());
-    std::vector<std::vector<double2>> eh(fin.size());
-    // preconditioned solves end with Y = al P in P (the batch is gated off
-    // for them, so the next chunk leaves it alone)
-    for (int f : fin)
-        if (slots_[f].prob->sys->jacobi()) slots_[f].prob->sys->scale_solution(false, main_);
-    if (opt_.keep_fields) {
-        for (size_t k = 0; k < fin.size(); ++k) {
-            Problem& p = *slots_[fin[k]].prob;
-            ebuf[k].alloc(3ull * p.ndip);
-            p.field(ebuf[k].get(), main_);
-            eh[k].resize(3ull * p.ndip);
-            ebuf[k].download(eh[k].data(), eh[k].size(), main_);
-        }
-        TDDA_HIP(hipStreamSynchronize(main_));
-    }
     const double t = now();
-    std::lock_guard<std::mutex> lk(mu_);
+    std::vector<Retiring> out(fin.size());
     for (size_t k = 0; k < fin.size(); ++k) {
         Slot& sl = slots_[fin[k]];
         const KScalars& ks = snap[fin[k]];
-        ServeResult r;
-        r.id = sl.req.id;
-        r.status = ks.status;
-        r.iterations = ks.iterations;
-        r.n_ru = ks.n_ru;
-        r.err = ks.err;
-        r.err_est = ks.err_est;
-        r.t_submit = sl.req.t_submit;
-        r.t_admit = sl.t_admit;
-        r.t_done = t;
-        r.op_hit = sl.op_hit;
-        done_.push_back(r);
-        if (opt_.keep_fields) fields_[r.id] = std::move(eh[k]);
-        const auto key = std::make_tuple(sl.req.desc.nx, sl.req.desc.ny, sl.req.desc.nz);
-        sl.prob->sys->op.reset();   // let the operator cache evict it
-        pool_[key].push_back(std::move(sl.prob->sys));
-        sl.prob.reset();
-        sl.used = false;
-        busy_--;
-        stats_.completed++;
+        Retiring& rt = out[k];
+        rt.r.id = sl.req.id;
+        rt.r.status = ks.status;
+        rt.r.iterations = ks.iterations;
+        rt.r.n_ru = ks.n_ru;
+        rt.r.err = ks.err;
+        rt.r.err_est = ks.err_est;
+        rt.r.t_submit = sl.req.t_submit;
+        rt.r.t_admit = sl.t_admit;
+        rt.r.op_hit = sl.op_hit;
+        rt.shape = std::make_tuple(sl.req.desc.nx, sl.req.desc.ny, sl.req.desc.nz);
+        rt.prob = std::move(sl.prob);
+        // a preconditioned solve ends with Y = al P in P (the batch is gated
+        // off for it, so later chunks leave it alone)
+        if (rt.prob->sys->jacobi()) rt.prob->sys->scale_solution(false, main_);
+        if (opt_.keep_fields) {
+            rt.n = 3ull * rt.prob->ndip;
+            rt.dev.alloc(rt.n);
+            rt.prob->field(rt.dev.get(), main_);
+            auto& pool = pinned_fields_[rt.n];
+            if (!pool.empty()) {
+                rt.host = pool.back();
+                pool.pop_back();
+            } else {
+                TDDA_HIP(hipHostMalloc((void**)&rt.host, rt.n * sizeof(double2), hipHostMallocDefault));
+            }
+            TDDA_HIP(hipMemcpyAsync(rt.host, rt.dev.get(), rt.n * sizeof(double2), hipMemcpyDeviceToHost, main_));
+            TDDA_HIP(hipEventCreateWithFlags(&rt.ev, hipEventDisableTiming));
+            TDDA_HIP(hipEventRecord(rt.ev, main_));
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (int f : fin) slots_[f].used = false;
+    }
+    for (auto& rt : out) {
+        if (opt_.keep_fields) retiring_.push_back(std::move(rt));
+        else finish(rt, t);
+    }
+}
+
+void Server::finish(Retiring& rt, double t) {
+    rt.r.t_done = t;
+    std::vector<double2> field;
+    if (rt.host) {
+        field.assign(rt.host, rt.host + rt.n);
+        pinned_fields_[rt.n].push_back(rt.host);
+        rt.host = nullptr;
+    }
+    if (rt.ev) (void)hipEventDestroy(rt.ev);
+    rt.ev = nullptr;
+    rt.dev.release();
+    rt.prob->sys->op.reset();   // let the operator cache evict it
+    pool_[rt.shape].push_back(std::move(rt.prob->sys));
+    rt.prob.reset();
+    std::lock_guard<std::mutex> lk(mu_);
+    done_.push_back(rt.r);
+    if (opt_.keep_fields) fields_[rt.r.id] = std::move(field);
+    busy_--;
+    stats_.completed++;
+}
+
+void Server::complete_retirements(bool wait) {
+    while (!retiring_.empty()) {
+        Retiring& rt = retiring_.front();
+        if (wait) {
+            TDDA_HIP(hipEventSynchronize(rt.ev));
+        } else {
+            const hipError_t e = hipEventQuery(rt.ev);
+            if (e == hipErrorNotReady) break;
+            TDDA_HIP(e);
+        }
+        finish(rt, now());
+        retiring_.pop_front();
+    }
+}
+
+void Server::drop_graph(int b) {
+    if (chunk_graph_[b]) (void)hipGraphExecDestroy(chunk_graph_[b]);
+    chunk_graph_[b] = nullptr;
+}
+
+// A chunk of iterations of batch b and the end-of-chunk update, on main_.
+void Server::enqueue_chunk(int b) {
+    const TwiddlePools tw = PlanRegistry::instance().pools();
+    const LaunchSet& ls = batch_[b].ls;
+    auto body = [&](hipStream_t st) {
+        for (int k = 0; k < opt_.chunk; ++k) launch_iteration(ls, tw, opt_.refined, st);
+        launch_finalize(ls, st);
+    };
+    batch_uses_[b]++;
+    if (graph_capture() && batch_uses_[b] >= 4) {   // recording pays back over a few replays
+        if (!chunk_graph_[b]) chunk_graph_[b] = capture_graph(body);
+        TDDA_HIP(hipGraphLaunch(chunk_graph_[b], main_));
+    } else {
+        body(main_);
     }
 }
 
@@ -223,14 +296,7 @@ void Server::loop() {
         while (true) {
             // 1. enqueue a chunk of iterations for batch `cur`
             if (!members_[cur].empty()) {
-                const TwiddlePools tw = PlanRegistry::instance().pools();
-                const LaunchSet& ls = batch_[cur].ls;
-                for (int k = 0; k < opt_.chunk; ++k) {
-                    launch_matvec(ls, tw, opt_.refined, XF_P, XI_V, GATE_ACTIVE, main_);
-                    launch_matvec(ls, tw, opt_.refined, XF_T, XI_S, GATE_ACTIVE, main_);
-                    if (opt_.refined) launch_matvec(ls, tw, false, XF_SOL, XI_RESID, GATE_RU, main_);
-                }
-                launch_finalize(ls, main_);
+                enqueue_chunk(cur);
                 TDDA_HIP(hipMemcpyAsync(snap_[cur], ks_.get(), opt_.max_slots * sizeof(KScalars),
                                         hipMemcpyDeviceToHost, main_));
                 TDDA_HIP(hipEventRecord(chunk_done_[cur], main_));
@@ -260,13 +326,18 @@ void Server::loop() {
                 stats_.peak_bytes = std::max(stats_.peak_bytes, device_bytes_in_use());
             }
             // jobs still in the running batch are gated off once finished,
-            // so their slots and buffers can be recycled now
+            // so their slots can be recycled now (their buffers once their
+            // field has been copied out)
             retire(fin, snap_[prev]);
-            // 4. build the other batch for the next chunk
+            complete_retirements(false);
+            // 4. the other batch for the next chunk: every live request (the
+            // live members of the current batch, which holds all admitted
+            // before it was built, and this round's joiners)
             std::vector<std::pair<int, unsigned>> next;
             for (const auto& m : members_[cur])
                 if (slots_[m.first].used && slots_[m.first].gen == m.second) next.push_back(m);
             if (!joined.empty()) {
+                live_version_++;
                 TDDA_HIP(hipStreamWaitEvent(main_, setup_done, 0));
                 std::vector<System*> js;
                 std::vector<KScalars*> jk;
@@ -288,22 +359,34 @@ void Server::loop() {
                 launch_init(init_batch_[prev].ls, false, main_);
                 for (int s : joined) next.emplace_back(s, slots_[s].gen);
             }
-            members_[prev] = next;
-            if (!next.empty()) {
-                std::vector<System*> sys;
-                std::vector<KScalars*> ks;
-                for (const auto& m : next) {
-                    sys.push_back(slots_[m.first].prob->sys.get());
-                    ks.push_back(ks_.get() + m.first);
+            size_t dead = 0;
+            for (const auto& m : members_[prev])
+                if (!slots_[m.first].used || slots_[m.first].gen != m.second) ++dead;
+            const bool stale = batch_version_[prev] != live_version_ || members_[prev].empty() ||
+                               2 * dead > members_[prev].size();
+            if (stale) {
+                members_[prev] = next;
+                drop_graph(prev);
+                batch_uses_[prev] = 0;
+                batch_version_[prev] = live_version_;
+                if (!next.empty()) {
+                    std::vector<System*> sys;
+                    std::vector<KScalars*> ks;
+                    for (const auto& m : next) {
+                        sys.push_back(slots_[m.first].prob->sys.get());
+                        ks.push_back(ks_.get() + m.first);
+                    }
+                    batch_[prev].build(sys, ks, opt_.refined, main_);
+                    std::lock_guard<std::mutex> lk(mu_);
+                    stats_.rebuilds++;
+                    stats_.max_active = std::max(stats_.max_active, (int)next.size());
                 }
-                batch_[prev].build(sys, ks, opt_.refined, main_);
-                std::lock_guard<std::mutex> lk(mu_);
-                stats_.rebuilds++;
-                stats_.max_active = std::max(stats_.max_active, (int)next.size());
             }
+            if (next.empty()) members_[prev].clear();   // nothing live: the loop idles
             cur = prev;
             // 5. idle: nothing running, queued or in flight
             if (members_[cur].empty() && !inflight_[0] && !inflight_[1]) {
+                complete_retirements(true);
                 std::unique_lock<std::mutex> lk(mu_);
                 if (stop_) break;
                 if (pending_.empty())
@@ -315,6 +398,7 @@ void Server::loop() {
             }
         }
         (void)hipStreamSynchronize(main_);
+        complete_retirements(true);
         (void)hipEventDestroy(setup_done);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);

@@ -25,7 +25,7 @@ struct ServerOptions {
     bool refined = true;             // c64 Krylov + FP64 reliable updates; false: FP64
     int maxit = 100000;
     double tol = 1e-5;
-    double ru_delta = 0.1;
+    double ru_delta = 0.01;   // reliable update when the residual falls this far below its peak since the last one
     double err_tol = 1e-5;           // refined: field-error target (0: true residual only)
     bool jacobi = false;             // right Jacobi preconditioning of every solve
     size_t hbm_budget = size_t(200) << 30;
@@ -84,6 +84,20 @@ class Server {
     void loop();
     std::vector<int> admit();
     void retire(const std::vector<int>& slots, const KScalars* snap);
+    // finished requests whose field is still being copied to the host
+    struct Retiring {
+        ServeResult r;
+        std::unique_ptr<Problem> prob;   // kept until the copy has read its buffers
+        std::tuple<int, int, int> shape;
+        DevBuf<double2> dev;
+        double2* host = nullptr;         // pinned
+        size_t n = 0;
+        hipEvent_t ev = nullptr;
+    };
+    std::deque<Retiring> retiring_;
+    std::map<size_t, std::vector<double2*>> pinned_fields_;   // pinned buffers by length
+    void complete_retirements(bool wait);
+    void finish(Retiring& rt, double t);
 
     ServerOptions opt_;
     SiciDevice tables_;
@@ -109,6 +123,17 @@ class Server {
     bool inflight_[2] = {false, false};
     std::vector<Slot> slots_;
     std::vector<std::pair<int, unsigned>> members_[2];   // (slot, generation) in batch i
+    // A batch holds every request admitted when it was built (finished ones
+    // stay in it, gated off, until it is rebuilt).  Admissions bump
+    // live_version_; a batch is rebuilt before use when its version is older or
+    // more than half its members have finished.  A batch used a fourth time
+    // without a rebuild has its chunk recorded as a graph and replayed.
+    long live_version_ = 0;
+    long batch_version_[2] = {-1, -1};
+    int batch_uses_[2] = {0, 0};
+    hipGraphExec_t chunk_graph_[2] = {nullptr, nullptr};
+    void enqueue_chunk(int b);
+    void drop_graph(int b);
     Batch batch_[2], init_batch_[2];
     // densities of an admission round go up through a pinned stage; two
     // stages alternate, each reused after the round two back has uploaded

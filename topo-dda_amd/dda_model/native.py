@@ -85,6 +85,7 @@ def lib():
         L.tdda_server_take_field.argtypes = [vp, ctypes.c_long, dp, ctypes.c_long]
         L.tdda_server_idle.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.tdda_set_graph_capture.argtypes = [ctypes.c_int]
         L.tdda_device_bytes.restype = ctypes.c_long
         L.tdda_trim_caches.restype = ctypes.c_int
         _LIB = L
@@ -252,6 +253,12 @@ def apply(problems, xs, c64=False):
     return out
 
 
+def set_graph_capture(on: bool):
+    """Record batched iterations as HIP graphs and replay them (default), or
+    issue the launches one by one."""
+    check(lib().tdda_set_graph_capture(int(bool(on))))
+
+
 def time_passes(problems, c64=True, reps=20):
     """Average ms per batched launch of (X fwd, Y fwd, Z, Y inv, X inv)."""
     arr = (ctypes.c_void_p * len(problems))(*[p.h for p in problems])
@@ -272,7 +279,7 @@ def time_iteration(problems, reps=20):
 class Server:
     """Continuous-batching request server (C++ loop thread)."""
 
-    def __init__(self, precision="fp32_refined", maxit=100000, tol=1e-5, ru_delta=0.1, hbm_budget_gb=200.0,
+    def __init__(self, precision="fp32_refined", maxit=100000, tol=1e-5, ru_delta=0.01, hbm_budget_gb=200.0,
                  max_slots=64, chunk=4, keep_fields=True, tables=None, err_tol=0.0, precond="none"):
         """err_tol: relative field-error target of refined solves (0: tol;
         negative: stop on the true residual alone).  precond: "none" or
