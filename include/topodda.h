@@ -258,6 +258,9 @@ typedef struct {
     long submitted, completed, chunks, rebuilds, launches;
     int max_active;
     long peak_bytes;
+    /* host time of the loop thread (ms): admissions, waits for a chunk,
+     * retirements, batch builds (maps and joins) */
+    double host_admit_ms, host_wait_ms, host_retire_ms, host_build_ms;
 } tdda_server_stats;
 
 int tdda_server_create(const tdda_server_opts* o, const double* si, const double* ci, long nsici, double delta,

@@ -584,6 +584,10 @@ int tdda_server_get_stats(tdda_server* s, tdda_server_stats* out) {
         out->launches = st.launches;
         out->max_active = st.max_active;
         out->peak_bytes = (long)st.peak_bytes;
+        out->host_admit_ms = st.host_ms[0];
+        out->host_wait_ms = st.host_ms[1];
+        out->host_retire_ms = st.host_ms[2];
+        out->host_build_ms = st.host_ms[3];
     });
 }
 

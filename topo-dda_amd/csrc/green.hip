@@ -4,7 +4,7 @@
 //    octant offsets o = (ox,oy,oz) >= 0 (AProductCore.cpp:63-244, 335-458).
 //    Every component is even or odd in each axis, and the image sum is
 //    symmetric in m -> -m, n -> -n, so the octant determines the full grid.
-// 2. k_green_dft (x3): the spectrum on the padded grid restricted to
+// 2. k_green_dft (x3, one axis each): the spectrum on the padded grid restricted to
 //    0 <= k <= L/2 per axis: Ghat(k) = G(0) + sum_{o>0} G(o) 2cos(2 pi k o/L)
 //    for an even axis, sum_{o>0} G(o) (-2i) sin(2 pi k o/L) for an odd one.
 //    This is the DFT of the zero-padded, wrapped grid the reference builds
@@ -80,59 +80,91 @@ __global__ void __launch_bounds__(256) k_green_lattice(GreenParams P, const doub
     }
 }
 
-// out[c][a][k][i] = sum_o W_c(k,o) in[c][a][o][i], 0 <= k < h, 0 <= o < no.
-// A thread computes GREEN_KB outputs k = kb + j nk of one column (j < KB,
-// nk = ceil(h / KB)), so every input it loads feeds KB products; the twiddle
-// table sits in LDS.  Consecutive threads take consecutive columns i (then
-// consecutive kb), so the input loads are coalesced along the contiguous axis
-// or shared by a wave when inner = 1.
-constexpr int GREEN_KB = 4;
-constexpr int GREEN_MAX_L = 256;
-__global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
-                                                   int outer, int no, int h, int inner, int L, int axis,
-                                                   const double2* __restrict__ tw) {
-    __shared__ double2 twl[GREEN_MAX_L];
-    for (int t = threadIdx.x; t < L; t += blockDim.x) twl[t] = tw[t];
-    __syncthreads();
-    const int nk = (h + GREEN_KB - 1) / GREEN_KB;
-    const long total = 6L * outer * nk * inner;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const int i = (int)(idx % inner);
-    const int kb = (int)((idx / inner) % nk);
-    const long a = (idx / ((long)inner * nk)) % outer;
-    const int c = (int)(idx / ((long)inner * nk * outer));
-    const bool odd = c_odd[c][axis] != 0;
-    const double2* src = in + ((long)c * outer + a) * no * inner + i;
-    int k[GREEN_KB], m[GREEN_KB];
-    double2 acc[GREEN_KB];
+// out[c][a][k][i] = sum_o W_c(k,o) in[c][a][o][i], 0 <= k < h, 0 <= o < no,
+// with W_c(k,0) = 1, W_c(k,o) = 2 cos(2 pi k o/L) for an even component and
+// W_c(k,0) = 0, W_c(k,o) = -i 2 sin(2 pi k o/L) for an odd one.
+// A thread owns one column (a, i) and KB consecutive outputs k0 .. k0+KB-1;
+// c and k0 are uniform over the workgroup, so the phase indices k o mod L and
+// the twiddles exp(-2 pi i m/L) are scalar values shared by the whole group,
+// and every input the thread loads feeds KB multiply-adds.  The sum over o
+// runs in order, o = 0 first.
+// The sum over o for one parity.  Half the sum is accumulated and doubled at
+// the end: scaling by 2 is exact, so this equals summing 2 cos / 2 sin terms
+// in the same order.  The extra outputs of the last group repeat k = h - 1
+// (so every phase stays in the table) and are not stored; lanes past the
+// last column read column 0 and store nothing.
+template <int KB, bool ODD>
+__device__ inline void green_dft_column(const double2* __restrict__ src, long stride, int no, int k0, int h, int L,
+                                        const double2* __restrict__ tw, double2 (&acc)[KB]) {
+    // byte offsets of the phases k o mod L in the table (of the sine parts
+    // for an odd component)
+    const unsigned period = 16u * (unsigned)L;
+    unsigned mb[KB];
+    const double2 v0 = src[0];
 #pragma unroll
-    for (int j = 0; j < GREEN_KB; ++j) {
-        k[j] = min(kb + j * nk, h - 1);   // clamped lanes compute a duplicate that is not stored
-        m[j] = 0;                         // phase index k o mod L, advanced by k per input
-        acc[j] = make_double2(0.0, 0.0);
+    for (int j = 0; j < KB; ++j) {
+        mb[j] = 16u * (unsigned)min(k0 + j, h - 1);   // phase at o = 1
+        acc[j] = ODD ? make_double2(0.0, 0.0) : make_double2(0.5 * v0.x, 0.5 * v0.y);   // o = 0, weight 1
     }
-    for (int o = 0; o < no; ++o) {
-        const double2 v = src[(long)o * inner];
+    const char* twb = reinterpret_cast<const char*>(tw) + (ODD ? 8 : 0);
+    auto step = [&](const double2 v) {
+        double w[KB];
 #pragma unroll
-        for (int j = 0; j < GREEN_KB; ++j) {
-            const double2 w = twl[m[j]];   // exp(-2 pi i m / L)
-            if (!odd) {
-                const double f = o == 0 ? 1.0 : 2.0 * w.x;   // 2 cos
-                acc[j].x += f * v.x;
-                acc[j].y += f * v.y;
-            } else if (o > 0) {
-                const double s2 = -2.0 * w.y;   // 2 sin; (-i s2) v
-                acc[j].x += s2 * v.y;
-                acc[j].y -= s2 * v.x;
+        for (int j = 0; j < KB; ++j) w[j] = *reinterpret_cast<const double*>(twb + mb[j]);   // exp(-2 pi i m / L)
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            if (!ODD) {   // cos v
+                acc[j].x += w[j] * v.x;
+                acc[j].y += w[j] * v.y;
+            } else {      // sin (-i v) = sin (v.y, -v.x), from w = -sin
+                acc[j].x -= w[j] * v.y;
+                acc[j].y += w[j] * v.x;
             }
-            m[j] = (m[j] + k[j] >= L) ? m[j] + k[j] - L : m[j] + k[j];
+            mb[j] += 16u * (unsigned)min(k0 + j, h - 1);
+            mb[j] -= mb[j] >= period ? period : 0u;
         }
+    };
+    // four inputs in flight per group
+    int o = 1;
+    for (; o + 3 < no; o += 4) {
+        double2 vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vv[u] = src[(long)(o + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) step(vv[u]);
     }
+    for (; o < no; ++o) step(src[(long)o * stride]);
+#pragma unroll
+    for (int j = 0; j < KB; ++j) acc[j] = make_double2(2.0 * acc[j].x, 2.0 * acc[j].y);
+}
+
+template <int KB>
+__global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
+                                                   int outer, int no, int h, int inner, int L, int axis, int ng,
+                                                   const double2* __restrict__ tw) {
+    const long ncols = (long)outer * inner;
+    const long nbc = (ncols + 255) / 256;
+    const long b = blockIdx.x;
+    const long cb = b % nbc;
+    const int g = (int)((b / nbc) % ng);
+    const int c = (int)(b / (nbc * ng));
+    const int k0 = g * KB;
+    const int nk = min(KB, h - k0);
+    const long col = cb * 256 + threadIdx.x;
+    const bool valid = col < ncols;
+    const long a = valid ? col / inner : 0;
+    const int i = valid ? (int)(col - a * inner) : 0;
+    const double2* src = in + ((long)c * outer + a) * no * inner + i;
+    double2 acc[KB];
+    if (c_odd[c][axis])
+        green_dft_column<KB, true>(src, inner, no, k0, h, L, tw, acc);
+    else
+        green_dft_column<KB, false>(src, inner, no, k0, h, L, tw, acc);
+    if (!valid) return;
     double2* dst = out + ((long)c * outer + a) * h * inner + i;
 #pragma unroll
-    for (int j = 0; j < GREEN_KB; ++j)
-        if (kb + j * nk < h) dst[(long)(kb + j * nk) * inner] = acc[j];
+    for (int j = 0; j < KB; ++j)
+        if (j < nk) dst[(long)(k0 + j) * inner] = acc[j];
 }
 
 __global__ void k_green_interleave(const double2* __restrict__ in, long rows, int hx, double2* __restrict__ out64,
@@ -159,10 +191,16 @@ void launch_green_octant(const GreenParams& P, const double* si, const double* c
 
 void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
                       const double2* tw, hipStream_t st) {
-    TDDA_REQUIRE(L <= GREEN_MAX_L, ERR_INVALID, "transform length beyond the Green's DFT table");
-    const long total = 6L * outer * ((h + GREEN_KB - 1) / GREEN_KB) * inner;
-    hipLaunchKernelGGL(k_green_dft, dim3((total + 255) / 256), dim3(256), 0, st, in, out, outer, no, h, inner, L,
-                       axis, tw);
+    // KB = 8 or 16 outputs per thread, whichever wastes fewer on the last group
+    const int g8 = (h + 7) / 8, g16 = (h + 15) / 16;
+    const bool small = 8 * g8 < 16 * g16;
+    const int ng = small ? g8 : g16;
+    const long nbc = ((long)outer * inner + 255) / 256;
+    const dim3 grid((unsigned)(6L * ng * nbc));
+    if (small)
+        hipLaunchKernelGGL(k_green_dft<8>, grid, dim3(256), 0, st, in, out, outer, no, h, inner, L, axis, ng, tw);
+    else
+        hipLaunchKernelGGL(k_green_dft<16>, grid, dim3(256), 0, st, in, out, outer, no, h, inner, L, axis, ng, tw);
     TDDA_HIP(hipGetLastError());
 }
 

@@ -89,6 +89,8 @@ void launch_matvec_timed(const LaunchSet& ls, const TwiddlePools& tw, bool c64, 
 void launch_finalize(const LaunchSet& ls, hipStream_t st);
 // Initial residual, r0, p = v = 0 and scalars; warm: r = rhs - yout.
 void launch_init(const LaunchSet& ls, bool warm, hipStream_t st);
+// Every job's scalars (GridJob::ks) set to t, in one launch.
+void launch_ks_template(const LaunchSet& ls, const KScalars& t, hipStream_t st);
 
 // The device counter behind LaunchSet::ru_jobs (one per process, zeroed).
 int* ru_job_counter();

@@ -612,6 +612,17 @@ void launch_finalize(const LaunchSet& ls, hipStream_t st) {
     TDDA_HIP(hipGetLastError());
 }
 
+__global__ void k_ks_template(const GridJob* __restrict__ jobs, int n, KScalars t) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) *jobs[j].ks = t;
+}
+
+void launch_ks_template(const LaunchSet& ls, const KScalars& t, hipStream_t st) {
+    if (ls.njobs == 0) return;
+    hipLaunchKernelGGL(k_ks_template, dim3((ls.njobs + 63) / 64), dim3(64), 0, st, ls.jobs, ls.njobs, t);
+    TDDA_HIP(hipGetLastError());
+}
+
 void launch_init(const LaunchSet& ls, bool warm, hipStream_t st) {
     if (ls.n_upd == 0) return;
     if (ls.refined)

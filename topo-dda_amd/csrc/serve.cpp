@@ -20,7 +20,6 @@ Server::Server(const ServerOptions& o, const SiciDevice& tables) : opt_(o), tabl
     }
     for (int i = 0; i < 2; ++i) {
         TDDA_HIP(hipHostMalloc((void**)&snap_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
-        TDDA_HIP(hipHostMalloc((void**)&init_[i], o.max_slots * sizeof(KScalars), hipHostMallocDefault));
         std::memset(snap_[i], 0, o.max_slots * sizeof(KScalars));
         TDDA_HIP(hipEventCreateWithFlags(&chunk_done_[i], hipEventDisableTiming));
         TDDA_HIP(hipEventCreateWithFlags(&rho_done_[i], hipEventDisableTiming));
@@ -28,6 +27,7 @@ Server::Server(const ServerOptions& o, const SiciDevice& tables) : opt_(o), tabl
     TDDA_HIP(hipStreamSynchronize(main_));
     slots_.resize(o.max_slots);
     th_ = std::thread([this] { loop(); });
+    adm_th_ = std::thread([this] { admit_loop(); });
 }
 
 Server::~Server() {
@@ -36,6 +36,8 @@ Server::~Server() {
         stop_ = true;
     }
     cv_.notify_all();
+    cv_adm_.notify_all();
+    if (adm_th_.joinable()) adm_th_.join();
     if (th_.joinable()) th_.join();
     (void)hipStreamSynchronize(main_);
     (void)hipStreamSynchronize(setup_);
@@ -48,11 +50,13 @@ Server::~Server() {
     retiring_.clear();
     for (auto& kv : pinned_fields_)
         for (double2* h : kv.second) (void)hipHostFree(h);
+    for (auto& jr : joining_) (void)hipEventDestroy(jr.ev);
+    joining_.clear();
+    for (hipEvent_t e : free_events_) (void)hipEventDestroy(e);
     slots_.clear();
     pool_.clear();
     for (int i = 0; i < 2; ++i) {
         if (snap_[i]) (void)hipHostFree(snap_[i]);
-        if (init_[i]) (void)hipHostFree(init_[i]);
         if (chunk_done_[i]) (void)hipEventDestroy(chunk_done_[i]);
         if (rho_done_[i]) (void)hipEventDestroy(rho_done_[i]);
     }
@@ -74,7 +78,7 @@ void Server::submit(ServeRequest&& r) {
         pending_.push_back(std::move(r));
         stats_.submitted++;
     }
-    cv_.notify_all();
+    cv_adm_.notify_all();
 }
 
 int Server::poll(std::vector<ServeResult>& out, int max) {
@@ -138,16 +142,20 @@ std::vector<int> Server::admit() {
             pending_.pop_front();
             busy_++;
             slots_[free_slot].used = true;
+            slots_[free_slot].gen++;
             joined.push_back(free_slot);
         }
+        // the slot is this thread's until its round is handed to the loop
         Slot& sl = slots_[joined.back()];
-        sl.gen++;
         const auto key = std::make_tuple(req.desc.nx, req.desc.ny, req.desc.nz);
         std::unique_ptr<System> reuse;
-        auto pit = pool_.find(key);
-        if (pit != pool_.end() && !pit->second.empty()) {
-            reuse = std::move(pit->second.back());
-            pit->second.pop_back();
+        {
+            std::lock_guard<std::mutex> lk(pool_mu_);
+            auto pit = pool_.find(key);
+            if (pit != pool_.end() && !pit->second.empty()) {
+                reuse = std::move(pit->second.back());
+                pit->second.pop_back();
+            }
         }
         sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
         sl.prob->sys->P.zero(setup_);
@@ -175,7 +183,68 @@ std::vector<int> Server::admit() {
     }
     TDDA_HIP(hipEventRecord(rho_done_[r], setup_));
     rho_pending_[r] = true;
+    // hand the round to the loop thread: it joins once this event completes
+    JoinRound jr;
+    jr.slots = joined;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!free_events_.empty()) {
+            jr.ev = free_events_.back();
+            free_events_.pop_back();
+        }
+    }
+    if (!jr.ev) TDDA_HIP(hipEventCreateWithFlags(&jr.ev, hipEventDisableTiming));
+    TDDA_HIP(hipEventRecord(jr.ev, setup_));
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        joining_.push_back(std::move(jr));
+    }
+    cv_.notify_all();
     return joined;
+}
+
+// Admission thread: admit whenever requests wait and a slot is free; when
+// memory holds admissions back, retry after a retirement (or 2 ms).
+void Server::admit_loop() {
+    try {
+        while (true) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_adm_.wait_for(lk, std::chrono::milliseconds(20), [this] {
+                    if (stop_ || !error_.empty()) return true;
+                    if (pending_.empty()) return false;
+                    for (const Slot& sl : slots_)
+                        if (!sl.used) return true;
+                    return false;
+                });
+                if (stop_ || !error_.empty()) break;
+            }
+            const double t0 = now();
+            const std::vector<int> admitted = admit();
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                stats_.host_ms[0] += 1e3 * (now() - t0);
+            }
+            if (admitted.empty()) {
+                std::unique_lock<std::mutex> lk(mu_);
+                if (stop_) break;
+                cv_adm_.wait_for(lk, std::chrono::milliseconds(2));
+            }
+        }
+    } catch (const std::exception& e) {
+        fail(e.what());
+    }
+}
+
+void Server::fail(const std::string& what) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (error_.empty()) error_ = what;
+        busy_ = 0;
+        pending_.clear();
+    }
+    cv_.notify_all();
+    cv_adm_.notify_all();
 }
 
 // Finished requests leave their slot at once.  Without kept fields they are
@@ -224,6 +293,7 @@ void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
         std::lock_guard<std::mutex> lk(mu_);
         for (int f : fin) slots_[f].used = false;
     }
+    cv_adm_.notify_all();
     for (auto& rt : out) {
         if (opt_.keep_fields) retiring_.push_back(std::move(rt));
         else finish(rt, t);
@@ -242,13 +312,19 @@ void Server::finish(Retiring& rt, double t) {
     rt.ev = nullptr;
     rt.dev.release();
     rt.prob->sys->op.reset();   // let the operator cache evict it
-    pool_[rt.shape].push_back(std::move(rt.prob->sys));
+    {
+        std::lock_guard<std::mutex> lk(pool_mu_);
+        pool_[rt.shape].push_back(std::move(rt.prob->sys));
+    }
     rt.prob.reset();
-    std::lock_guard<std::mutex> lk(mu_);
-    done_.push_back(rt.r);
-    if (opt_.keep_fields) fields_[rt.r.id] = std::move(field);
-    busy_--;
-    stats_.completed++;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_.push_back(rt.r);
+        if (opt_.keep_fields) fields_[rt.r.id] = std::move(field);
+        busy_--;
+        stats_.completed++;
+    }
+    cv_adm_.notify_all();   // memory was freed
 }
 
 void Server::complete_retirements(bool wait) {
@@ -264,6 +340,32 @@ void Server::complete_retirements(bool wait) {
         finish(rt, now());
         retiring_.pop_front();
     }
+}
+
+// (the admission thread only appends rounds, this thread only removes them)
+std::vector<int> Server::ready_joiners(bool wait) {
+    std::vector<int> ready;
+    while (true) {
+        hipEvent_t ev = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (joining_.empty()) break;
+            ev = joining_.front().ev;
+        }
+        if (wait && ready.empty()) {
+            TDDA_HIP(hipEventSynchronize(ev));
+        } else {
+            const hipError_t e = hipEventQuery(ev);
+            if (e == hipErrorNotReady) break;
+            TDDA_HIP(e);
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        JoinRound& jr = joining_.front();
+        ready.insert(ready.end(), jr.slots.begin(), jr.slots.end());
+        free_events_.push_back(jr.ev);
+        joining_.pop_front();
+    }
+    return ready;
 }
 
 void Server::drop_graph(int b) {
@@ -290,8 +392,6 @@ void Server::enqueue_chunk(int b) {
 
 void Server::loop() {
     try {
-        hipEvent_t setup_done;
-        TDDA_HIP(hipEventCreateWithFlags(&setup_done, hipEventDisableTiming));
         int cur = 0;   // batch whose chunk is enqueued next
         while (true) {
             // 1. enqueue a chunk of iterations for batch `cur`
@@ -305,22 +405,23 @@ void Server::loop() {
                 stats_.chunks++;
                 stats_.launches += (long)opt_.chunk * (opt_.refined ? 18 : 12) + 3;
             }
-            // 2. admit arrivals (operators and densities on the setup stream)
-            std::vector<int> joined = admit();
-            if (!joined.empty()) TDDA_HIP(hipEventRecord(setup_done, setup_));
-            // 3. statuses of the previous chunk (the current one keeps running)
+            // 2. statuses of the previous chunk (the current one keeps running)
+            double tm[4];
+            tm[0] = now();
             const int prev = 1 - cur;
-            // (a slot retired after an earlier chunk may already hold a new
-            // request: only the generation that ran in that chunk counts)
             std::vector<int> fin;
             if (inflight_[prev]) {
                 TDDA_HIP(hipEventSynchronize(chunk_done_[prev]));
                 inflight_[prev] = false;
+                // (a slot retired after an earlier chunk may already hold a
+                // new request: only the generation that ran in that chunk counts)
+                std::lock_guard<std::mutex> lk(mu_);
                 for (const auto& m : members_[prev])
                     if (slots_[m.first].used && slots_[m.first].gen == m.second &&
                         snap_[prev][m.first].status != JOB_ACTIVE)
                         fin.push_back(m.first);
             }
+            tm[1] = now();
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 stats_.peak_bytes = std::max(stats_.peak_bytes, device_bytes_in_use());
@@ -330,38 +431,43 @@ void Server::loop() {
             // field has been copied out)
             retire(fin, snap_[prev]);
             complete_retirements(false);
-            // 4. the other batch for the next chunk: every live request (the
-            // live members of the current batch, which holds all admitted
-            // before it was built, and this round's joiners)
+            tm[2] = now();
+            // 3. the other batch for the next chunk: every live request (the
+            // live members of the current batch, which holds all joined
+            // before it was built) and the admitted requests whose set-up has
+            // completed (with nothing else to run, the oldest round is waited for)
             std::vector<std::pair<int, unsigned>> next;
-            for (const auto& m : members_[cur])
-                if (slots_[m.first].used && slots_[m.first].gen == m.second) next.push_back(m);
+            size_t dead = 0;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                for (const auto& m : members_[cur])
+                    if (slots_[m.first].used && slots_[m.first].gen == m.second) next.push_back(m);
+                for (const auto& m : members_[prev])
+                    if (!slots_[m.first].used || slots_[m.first].gen != m.second) ++dead;
+            }
+            const std::vector<int> joined = ready_joiners(next.empty());
             if (!joined.empty()) {
                 live_version_++;
-                TDDA_HIP(hipStreamWaitEvent(main_, setup_done, 0));
                 std::vector<System*> js;
                 std::vector<KScalars*> jk;
-                for (size_t k = 0; k < joined.size(); ++k) {
-                    KScalars& in = init_[prev][k];
-                    std::memset(&in, 0, sizeof(KScalars));
-                    in.tol = opt_.tol;
-                    in.maxit = opt_.maxit;
-                    in.refined = opt_.refined ? 1 : 0;
-                    in.ru_delta = opt_.ru_delta;
-                    in.tol_e = opt_.refined ? opt_.err_tol : 0.0;
-                    in.status = JOB_ACTIVE;
-                    TDDA_HIP(hipMemcpyAsync(ks_.get() + joined[k], &in, sizeof(KScalars), hipMemcpyHostToDevice,
-                                            main_));
-                    js.push_back(slots_[joined[k]].prob->sys.get());
-                    jk.push_back(ks_.get() + joined[k]);
+                for (int s : joined) {
+                    js.push_back(slots_[s].prob->sys.get());
+                    jk.push_back(ks_.get() + s);
                 }
+                KScalars in;
+                std::memset(&in, 0, sizeof(KScalars));
+                in.tol = opt_.tol;
+                in.maxit = opt_.maxit;
+                in.refined = opt_.refined ? 1 : 0;
+                in.ru_delta = opt_.ru_delta;
+                in.tol_e = opt_.refined ? opt_.err_tol : 0.0;
+                in.status = JOB_ACTIVE;
                 init_batch_[prev].build(js, jk, opt_.refined, main_, false);
+                launch_ks_template(init_batch_[prev].ls, in, main_);
                 launch_init(init_batch_[prev].ls, false, main_);
+                std::lock_guard<std::mutex> lk(mu_);
                 for (int s : joined) next.emplace_back(s, slots_[s].gen);
             }
-            size_t dead = 0;
-            for (const auto& m : members_[prev])
-                if (!slots_[m.first].used || slots_[m.first].gen != m.second) ++dead;
             const bool stale = batch_version_[prev] != live_version_ || members_[prev].empty() ||
                                2 * dead > members_[prev].size();
             if (stale) {
@@ -383,28 +489,28 @@ void Server::loop() {
                 }
             }
             if (next.empty()) members_[prev].clear();   // nothing live: the loop idles
-            cur = prev;
-            // 5. idle: nothing running, queued or in flight
-            if (members_[cur].empty() && !inflight_[0] && !inflight_[1]) {
-                complete_retirements(true);
-                std::unique_lock<std::mutex> lk(mu_);
-                if (stop_) break;
-                if (pending_.empty())
-                    cv_.wait_for(lk, std::chrono::milliseconds(50), [this] { return stop_ || !pending_.empty(); });
-                if (stop_) break;
-            } else {
+            tm[3] = now();
+            {
                 std::lock_guard<std::mutex> lk(mu_);
-                if (stop_) break;
+                for (int q = 0; q < 3; ++q) stats_.host_ms[q + 1] += 1e3 * (tm[q + 1] - tm[q]);
+            }
+            cur = prev;
+            // 4. idle: nothing running, in flight or being set up
+            std::unique_lock<std::mutex> lk(mu_);
+            if (stop_ || !error_.empty()) break;
+            if (members_[cur].empty() && !inflight_[0] && !inflight_[1] && joining_.empty()) {
+                lk.unlock();
+                complete_retirements(true);
+                lk.lock();
+                cv_.wait_for(lk, std::chrono::milliseconds(50),
+                             [this] { return stop_ || !error_.empty() || !joining_.empty(); });
+                if (stop_ || !error_.empty()) break;
             }
         }
         (void)hipStreamSynchronize(main_);
         complete_retirements(true);
-        (void)hipEventDestroy(setup_done);
     } catch (const std::exception& e) {
-        std::lock_guard<std::mutex> lk(mu_);
-        error_ = e.what();
-        busy_ = 0;
-        pending_.clear();
+        fail(e.what());
     }
 }
 

@@ -1,11 +1,12 @@
 // Request server: continuous batching of DDA solves of uneven shape on one GPU.
 //
 // Requests (box shape, wavelength, incidence, density) are queued by submit().
-// A loop thread admits them in arrival order while a slot is free and the
-// device-memory estimate fits the budget, builds (or reuses from the cache)
-// their Green's spectra on a setup stream while the compute stream runs the
-// current batch, and advances every admitted solve together: each launch of
-// the five FFT passes and the Krylov kernels covers all active requests.
+// An admission thread takes them in arrival order while a slot is free and
+// the device-memory estimate fits the budget, and builds (or reuses from the
+// cache) their Green's spectra and per-dipole data on a setup stream.  A loop
+// thread runs the compute stream: requests join its batch once their set-up
+// has completed, and every joined solve advances together (each launch of
+// the five FFT passes and the Krylov kernels covers all active requests).
 // Finished requests leave the batch after every chunk of iterations; their
 // slots and device buffers are reused by the next arrivals.
 #pragma once
@@ -57,6 +58,9 @@ struct ServerStats {
     long submitted = 0, completed = 0, chunks = 0, rebuilds = 0, launches = 0;
     int max_active = 0;
     size_t peak_bytes = 0;
+    // host ms: admission thread (admit), loop thread (wait for a chunk,
+    // retire, join and rebuild)
+    double host_ms[4] = {0, 0, 0, 0};
 };
 
 class Server {
@@ -82,7 +86,9 @@ class Server {
         int op_hit = 0;
     };
     void loop();
+    void admit_loop();
     std::vector<int> admit();
+    void fail(const std::string& what);
     void retire(const std::vector<int>& slots, const KScalars* snap);
     // finished requests whose field is still being copied to the host
     struct Retiring {
@@ -102,9 +108,13 @@ class Server {
     ServerOptions opt_;
     SiciDevice tables_;
     std::chrono::steady_clock::time_point t0_;
-    std::thread th_;
+    std::thread th_, adm_th_;
+    // mu_ guards the queues, results, statistics, the slots' used flags and
+    // generations, the join rounds and the event pool; cv_ wakes the loop
+    // thread (a join round is ready or stop), cv_adm_ the admission thread
+    // (a request arrived, a slot or memory was freed, or stop)
     std::mutex mu_;
-    std::condition_variable cv_;
+    std::condition_variable cv_, cv_adm_;
     std::deque<ServeRequest> pending_;
     std::deque<ServeResult> done_;
     std::map<long, std::vector<double2>> fields_;
@@ -113,12 +123,11 @@ class Server {
     ServerStats stats_;
 
     // loop-thread state.  Two batches alternate: while the chunk of one runs
-    // on the GPU, the host retires what the previous chunk finished, admits
-    // arrivals and builds the other batch for the next chunk.
+    // on the GPU, the host retires what the previous chunk finished, joins
+    // admitted requests and builds the other batch for the next chunk.
     hipStream_t main_ = nullptr, setup_ = nullptr;
     DevBuf<KScalars> ks_;
     KScalars* snap_[2] = {nullptr, nullptr};   // pinned copies of ks_ after each chunk
-    KScalars* init_[2] = {nullptr, nullptr};   // pinned initial scalars of joining slots
     hipEvent_t chunk_done_[2] = {nullptr, nullptr};   // chunk of batch i complete (snap_[i] valid)
     bool inflight_[2] = {false, false};
     std::vector<Slot> slots_;
@@ -135,12 +144,28 @@ class Server {
     void enqueue_chunk(int b);
     void drop_graph(int b);
     Batch batch_[2], init_batch_[2];
-    // densities of an admission round go up through a pinned stage; two
-    // stages alternate, each reused after the round two back has uploaded
+    // Admitted requests whose operators and densities are still being set up
+    // on the setup stream, by admission round.  A round joins the batch once
+    // its event has completed, so the compute stream never waits for set-up
+    // while other requests can run.
+    struct JoinRound {
+        std::vector<int> slots;
+        hipEvent_t ev = nullptr;
+    };
+    std::deque<JoinRound> joining_;
+    std::vector<hipEvent_t> free_events_;
+    // slots of the rounds whose set-up is done (waits for the oldest when
+    // nothing else could run)
+    std::vector<int> ready_joiners(bool wait);
+    // admission-thread state: densities of an admission round go up through
+    // a pinned stage; two stages alternate, each reused after the round two
+    // back has uploaded
     PinnedStage rho_stage_[2];
     hipEvent_t rho_done_[2] = {nullptr, nullptr};
     bool rho_pending_[2] = {false, false};
     int rho_round_ = 0;
+    // systems of finished requests by box shape, reused by admissions
+    std::mutex pool_mu_;
     std::map<std::tuple<int, int, int>, std::vector<std::unique_ptr<System>>> pool_;
     int busy_ = 0;   // requests admitted but not finished (guarded by mu_)
 };
