@@ -144,6 +144,39 @@ int tdda_model_time_passes(tdda_model* m, int reps, double* ms5);
  * (on = 1, the default); 0 issues the same launches one by one.  Results are
  * the same either way.  No reference counterpart. */
 int tdda_set_graph_capture(int on);
+/* c64 transforms of boxes whose y and z lengths allow it can run the y, z
+ * and inverse y passes as one kernel per x frequency.  mode: TDDA_FUSED_OFF
+ * never, TDDA_FUSED_SCHEDULE where the kernel schedule enables it for the
+ * box's (Ly, Lz) (the default), TDDA_FUSED_ALL always.  Applies to batches
+ * built afterwards; results agree to c64 rounding.  No reference counterpart. */
+#define TDDA_FUSED_OFF 0
+#define TDDA_FUSED_SCHEDULE 1
+#define TDDA_FUSED_ALL 2
+int tdda_set_fused_yz(int mode);
+
+/* Kernel schedule: the split variant of each length-specialised pass per
+ * transform length and precision, and the fused y/z choice per (Ly, Lz).
+ * Text, one entry per line: "var <c64|c128> <xinv|y|z|xfwd> <L> <variant>",
+ * "yz <Ly> <Lz> <0|1>"; absent entries mean variant 0 and three passes.  It
+ * is read when systems are made (and batches built), and loaded at start-up
+ * from the file named by the environment variable TDDA_SCHEDULE.  Text
+ * outputs are NUL-terminated and cut to cap bytes; *len receives the full
+ * length.  No reference counterpart (the reference has one cuFFT plan per
+ * model). */
+int tdda_schedule_text(char* buf, long cap, long* len);
+int tdda_schedule_parse(const char* text);   /* replaces every entry; a malformed line leaves it unchanged */
+int tdda_schedule_load(const char* path);
+int tdda_schedule_save(const char* path);
+int tdda_schedule_clear(void);
+/* Autotuner: for count LDR problems of each of the n boxes (nx, ny, nz
+ * triples), times every split variant of every pass of the c64 and c128
+ * matvecs (reps batched launches each) and the c64 matvec with the fused y/z
+ * pass against three passes, records the faster choices in the schedule, and
+ * writes one report line per decision ("pass <prec> <pass> <L> ms <v0> <v1|->
+ * chosen <v>", "yz <Ly> <Lz> ms <three> <fused> chosen <0|1>").  Run it while
+ * no other work is being set up: the variants it tries apply to every system
+ * made meanwhile. */
+int tdda_tune(const int* boxes, int n, int count, int reps, char* report, long cap, long* len);
 /* Bytes of device memory currently held by the library. */
 long tdda_device_bytes(void);
 /* Drop cached Green's spectra nobody uses and return cached device blocks to

@@ -86,3 +86,35 @@ def test_generate_geometry_matches_oracle():
     import dda_model
     import dda_oracle as O
     assert np.array_equal(dda_model._generate_geometry(5, 3, 2), O.generate_geometry(5, 3, 2))
+
+
+def test_schedule_text_format_and_variants(tmp_path):
+    """The kernel schedule through the C ABI (host only): parse, text, save,
+    load, clear; variants exist only where the two split tables differ
+    (Z at 40, 48, 64; X and Y at 80, 96, 128); a bad line leaves the schedule
+    as it was."""
+    from dda_model import native
+    native.schedule_clear()
+    try:
+        native.schedule_set("# tuned\nvar c64 z 64 1\nvar c128 xfwd 128 1  # trailing comment\nyz 128 64 1\n")
+        t = native.schedule_text()
+        assert "var c64 z 64 1" in t and "var c128 xfwd 128 1" in t and "yz 128 64 1" in t
+        path = tmp_path / "schedule.txt"
+        native.schedule_save(path)
+        native.schedule_clear()
+        assert "var " not in native.schedule_text()
+        native.schedule_load(path)
+        assert native.schedule_text() == t
+        for bad in ["var c64 z 32 1",          # 32: both tables hold 8 x 4
+                    "var c64 z 64 2",          # no variant 2
+                    "var c64 y 100 1",         # no split at all
+                    "var c32 z 64 1", "var c64 w 64 1", "yz 128 64 3", "yz 128", "tune 1 2 3"]:
+            with pytest.raises(RuntimeError):
+                native.schedule_set(bad)
+            assert native.schedule_text() == t, bad
+        for ok in ["var c64 z 40 1", "var c64 z 48 1", "var c128 xinv 80 1", "var c64 y 96 1", "var c64 xfwd 128 0"]:
+            native.schedule_set(ok)
+        with pytest.raises(RuntimeError):
+            native.schedule_load(tmp_path / "missing.txt")
+    finally:
+        native.schedule_clear()

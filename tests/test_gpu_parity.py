@@ -393,3 +393,57 @@ def test_c64_apply_matches_fp64(dm, O, tables):
          f"{[round(i['error'], 8) for i in infos['fp32_refined']]}")
     for a, b in zip(its["fp32_refined"], its["fp64"]):
         assert 0.8 * b <= a <= 1.5 * b
+
+
+# every class of the fused y/z pass that the serving mix uses, odd sizes in a
+# class (ny = 47 -> Ly 96, nz = 31 -> Lz 64), and boxes outside it (Lx = 72
+# has no specialised X pass; Ly = 80 is no multiple of 16) in the same batch
+FUSED_SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 16), (48, 48, 24), (64, 64, 32), (20, 47, 31),
+                (40, 46, 16), (33, 31, 24), (20, 40, 16)]
+
+
+def test_fused_yz_apply_matches_three_passes(dm, O, tables):
+    """The fused c64 y/z pass (one workgroup per x frequency) against the
+    three-pass c64 path and the FP64 apply on the same operators, in one
+    mixed batch; then refined solves with and without it."""
+    from dda_model import native
+    _log("[gpu] fused y/z apply " + str(FUSED_SHAPES))
+    rng = np.random.default_rng(21)
+    probs, reqs = [], []
+    for i, s in enumerate(FUSED_SHAPES):
+        req = native.Request(shape=s, d=15.0, lam=450.0 + 37.0 * i, rho=rng.uniform(0, 1, int(np.prod(s))),
+                             n_k=(0.0, 0.0, 1.0), n_e0=(0.0, 1.0, 0.0))
+        reqs.append(req)
+        probs.append(native.Problem(req, precision="fp32_refined"))
+    xs = [rng.standard_normal(3 * p.n) + 1j * rng.standard_normal(3 * p.n) for p in probs]
+    y64 = native.apply(probs, xs, c64=False)
+    try:
+        native.set_fused_yz("off")
+        y3 = native.apply(probs, xs, c64=True)
+        native.set_fused_yz("all")
+        yf = native.apply(probs, xs, c64=True)
+    finally:
+        native.set_fused_yz("schedule")
+    for s, a, b, c in zip(FUSED_SHAPES, yf, y3, y64):
+        ef, e3, d = _rel(a, c), _rel(b, c), _rel(a, b)
+        _log(f"[gpu] {s}: fused vs fp64 {ef:.3e}, three-pass vs fp64 {e3:.3e}, fused vs three-pass {d:.3e}")
+        assert ef < 2e-6 and e3 < 2e-6 and d < 2e-6
+    box = O.OracleFullBox(FUSED_SHAPES[5], 15.0, reqs[5].lam, reqs[5].rho, workers=8)
+    assert _rel(y64[5], box.apply(xs[5])) < 1e-11
+    # refined solves through the fused pass reach the same fields as through
+    # the three passes (each within its 1e-5 field-error target) in about as
+    # many iterations
+    out = {}
+    for fused in (False, True):
+        try:
+            native.set_fused_yz("all" if fused else "off")
+            ps = [native.Problem(r, precision="fp32_refined") for r in reqs[:6]]
+            infos = native.solve(ps, tol=1e-5)
+            out[fused] = ([i["iterations"] for i in infos], [p.field() for p in ps], infos)
+        finally:
+            native.set_fused_yz("schedule")
+    _log(f"[gpu] refined iterations three-pass {out[False][0]} fused {out[True][0]}")
+    for a, b, ea, eb, ia in zip(out[True][0], out[False][0], out[True][1], out[False][1], out[True][2]):
+        assert ia["status"] == "converged" and ia["error"] <= 1e-5
+        assert abs(a - b) <= max(2, 0.15 * b)
+        assert _rel(ea, eb) < 2e-5

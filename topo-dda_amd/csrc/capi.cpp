@@ -1,10 +1,13 @@
 // extern "C" entry points (include/topodda.h) over the Model runtime.
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <string>
 
 #include "../../include/topodda.h"
 #include "model.h"
 #include "noalloc.h"
+#include "schedule.h"
 #include "objective.h"
 #include "serve.h"
 
@@ -48,6 +51,58 @@ long tdda_device_bytes(void) { return (long)tdda::device_bytes_in_use(); }
 
 int tdda_set_graph_capture(int on) {
     return guarded([&] { tdda::set_graph_capture(on != 0); });
+}
+
+int tdda_set_fused_yz(int mode) {
+    return guarded([&] { tdda::set_fused_yz(mode); });
+}
+
+static void copy_text(const std::string& t, char* buf, long cap, long* len) {
+    if (len) *len = (long)t.size();
+    if (buf && cap > 0) {
+        const size_t n = std::min((size_t)(cap - 1), t.size());
+        std::memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+}
+
+int tdda_schedule_text(char* buf, long cap, long* len) {
+    return guarded([&] { copy_text(tdda::Schedule::instance().text(), buf, cap, len); });
+}
+
+int tdda_schedule_parse(const char* text) {
+    return guarded([&] {
+        TDDA_REQUIRE(text != nullptr, tdda::ERR_INVALID, "null schedule text");
+        tdda::Schedule::instance().parse(text);
+    });
+}
+
+int tdda_schedule_load(const char* path) {
+    return guarded([&] {
+        TDDA_REQUIRE(path != nullptr, tdda::ERR_INVALID, "null path");
+        tdda::Schedule::instance().load(path);
+    });
+}
+
+int tdda_schedule_save(const char* path) {
+    return guarded([&] {
+        TDDA_REQUIRE(path != nullptr, tdda::ERR_INVALID, "null path");
+        tdda::Schedule::instance().save(path);
+    });
+}
+
+int tdda_schedule_clear(void) {
+    return guarded([&] { tdda::Schedule::instance().clear(); });
+}
+
+int tdda_tune(const int* boxes, int n, int count, int reps, char* report, long cap, long* len) {
+    return guarded([&] {
+        TDDA_REQUIRE(boxes != nullptr && n >= 1, tdda::ERR_INVALID, "no boxes to tune for");
+        std::vector<std::array<int, 3>> b(n);
+        for (int i = 0; i < n; ++i) b[i] = {boxes[3 * i], boxes[3 * i + 1], boxes[3 * i + 2]};
+        const tdda::TuneReport r = tdda::tune(b, count, reps, tdda::default_stream());
+        copy_text(tdda::tune_report_text(r), report, cap, len);
+    });
 }
 
 int tdda_trim_caches(void) {

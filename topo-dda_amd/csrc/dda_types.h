@@ -76,6 +76,11 @@ struct GridJob {
     int bxf32, bxf64, txf32, txf64;   // rows per tile / tiles of the X forward pass
     int tupd;                  // tiles of the elementwise update kernel
     int fx, fy, fz;            // 1: the pass uses the length-specialised kernels
+    int var[2][4];             // schedule variant per [c64, c128][PassKind] (with_split_pass)
+    // fused y/z pass of the c64 matvec: its length class (-1: none); tr32 = 1
+    // in a batch that runs this job through it (the X passes then keep the
+    // work grid as [kx][3][nz][ny])
+    int yzc, tr32;
     unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients
     const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
     // operator data

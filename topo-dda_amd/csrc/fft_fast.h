@@ -558,16 +558,44 @@ TDDA_HD bool with_split_z(int L, F&& f) {
     }
 }
 
+// Schedule variants.  Variant 0 of a pass uses its table above, variant 1 the
+// other table (X forward: with_split_xy / with_split; X inverse and Y:
+// with_split / with_split_xy; Z: with_split_z / with_split).  A length has a
+// variant 1 only where both tables hold a split and the splits differ, so a
+// length is specialised under either variant.  The autotuner times both and
+// records the faster per length, pass and precision.
+// PASS: 0 X inverse, 1 Y, 2 Z, 3 X forward (the PassKind values).
+template <int PASS, typename F>
+TDDA_HD bool with_split_pass(int L, int v, F&& f) {
+    if constexpr (PASS == 3) return v ? with_split(L, f) : with_split_xy(L, f);
+    else if constexpr (PASS == 2) return v ? with_split(L, f) : with_split_z(L, f);
+    else return v ? with_split_xy(L, f) : with_split(L, f);
+}
+
+// 2 when the pass has a variant 1 at length L, else 1
+inline int split_variants(int pass, int L) {
+    int a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+    auto ga = [&](auto s1, auto s2) { a1 = decltype(s1)::value; a2 = decltype(s2)::value; };
+    auto gb = [&](auto s1, auto s2) { b1 = decltype(s1)::value; b2 = decltype(s2)::value; };
+    bool ha = false, hb = false;
+    switch (pass) {
+        case 3: ha = with_split_xy(L, ga); hb = with_split(L, gb); break;
+        case 2: ha = with_split_z(L, ga); hb = with_split(L, gb); break;
+        default: ha = with_split(L, ga); hb = with_split_xy(L, gb); break;
+    }
+    return (ha && hb && (a1 != b1 || a2 != b2)) ? 2 : 1;
+}
+
 // Z lengths fall in two kernel classes: 0 for the splits with L2 = 4, 1 for
 // the others (their multiply step holds 3 x L2 >= 15 values per thread).
 // Each class is its own kernel, so the register budget of the short splits
 // is not set by the long ones.
 template <typename S2>
 constexpr int z_class_of() { return S2::value == 4 ? 0 : 1; }
-// -1: no specialised split
-inline int z_class(int L) {
+// -1: no specialised split (v: the schedule variant of the Z pass)
+inline int z_class(int L, int v = 0) {
     int c = -1;
-    with_split_z(L, [&](auto, auto s2) { c = z_class_of<decltype(s2)>(); });
+    with_split_pass<2>(L, v, [&](auto, auto s2) { c = z_class_of<decltype(s2)>(); });
     return c;
 }
 

@@ -87,7 +87,7 @@ void apply_noalloc(System& s, const double2* x, double2* y, hipStream_t st) {
         pm.n = tiles[k];
         pm.lds = lds_bytes(J, k, false);
         if (k == PASS_Z && J.fz) {
-            const int zc = z_class(J.lz);
+            const int zc = z_class(J.lz, J.var[1][PASS_Z]);
             pm.cls = 1 << zc;
             pm.lds_cls[zc] = pm.lds;
         }

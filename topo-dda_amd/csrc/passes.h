@@ -42,8 +42,13 @@ struct PassMap {
 // PASS_X: X inverse tiles (its dot partials are per tile); PASS_XF: X forward
 // tiles, sized apart from the inverse.
 enum PassKind : int { PASS_X = 0, PASS_Y = 1, PASS_Z = 2, PASS_XF = 3 };
+static_assert(PASS_X == 0 && PASS_Y == 1 && PASS_Z == 2 && PASS_XF == 3, "with_split_pass numbers the passes so");
 constexpr int NPASS = 4;
 constexpr int RU_REGIONS = 2 * NPASS;
+
+// Length classes of the fused y/z pass (c64): Ly = 16 LY1 with LY1 in
+// {2, 3, 4, 6, 8}, Lz in {32, 48, 64}.
+constexpr int NYZ = 15;
 
 // A launch over a batch: device array of jobs and, per transform precision
 // and pass, the blocks of the generic kernels and of the length-specialised
@@ -67,6 +72,9 @@ struct LaunchSet {
     const int2* map_upd = nullptr;
     int n_upd = 0;
     bool refined = false;            // Krylov vectors are c64
+    // fused y/z pass of the c64 matvec, one map per length class; its jobs
+    // are in no c64 Y or Z map
+    PassMap yz[NYZ];
 };
 
 // Twiddle pools (device), indexed by FftPlan::tw_off
@@ -95,7 +103,12 @@ void launch_ks_template(const LaunchSet& ls, const KScalars& t, hipStream_t st);
 // The device counter behind LaunchSet::ru_jobs (one per process, zeroed).
 int* ru_job_counter();
 
-// Host-side tile geometry for a job (fills the b*/t*/f* fields).
+// The tuned schedule (runtime.cpp): variant of a pass at length L for
+// precision pr (0 c64, 1 c128; 0 when none is recorded), and whether the
+// fused y/z pass is enabled for (Ly, Lz) (off when none is recorded).
+int schedule_variant(int pr, int pass, int L);
+bool schedule_fused(int ly, int lz);
+// Host-side tile geometry for a job (fills the b*/t*/f*/var fields).
 void plan_tiles(GridJob& J);
 // Dynamic LDS bytes of the job's tile in a pass (generic or specialised).
 size_t lds_bytes(const GridJob& J, int pass, bool c64);
@@ -109,9 +122,20 @@ void launch_fast_y(const PassMap& pm, const GridJob* jobs, const TwiddlePools& t
                    hipStream_t st);
 void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& tw, bool c64, int gate,
                    hipStream_t st);
+// Fused y/z pass (passes_yz.hip): class of a job's lengths (-1: none), LDS
+// bytes of a class, and the launches of every non-empty class map.
+int yz_class(int ly, int lz, int ny, int nz);
+size_t yz_lds_bytes(int cls);
+void launch_fast_yz(const PassMap* yz, const GridJob* jobs, const TwiddlePools& tw, int gate, hipStream_t st);
+// Which eligible c64 jobs batches built from now on run through the fused
+// y/z pass: FUSED_OFF none, FUSED_SCHEDULE those whose (Ly, Lz) the schedule
+// enables (the default), FUSED_ALL all.
+enum FusedMode : int { FUSED_OFF = 0, FUSED_SCHEDULE = 1, FUSED_ALL = 2 };
+void set_fused_yz(int mode);
+int fused_yz();
 // Rows (X) or pencils (Y, Z) per tile and LDS bytes of the specialised path
 // for length L; false if L has no specialised split.
-bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes);
+bool fast_tiles(int L, int pass, bool c64, int v, int* per_tile, size_t* lds_bytes);
 constexpr int UPD_EPT = 4;   // elements per thread in the update kernel
 
 }  // namespace tdda

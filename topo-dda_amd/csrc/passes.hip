@@ -15,6 +15,7 @@
 // launch covers a batch of jobs; block b handles (job, tile) = map[b].  Tiles
 // depend only on a job's own shape, and reductions are per job in a fixed
 // order, so a job's result does not depend on what else is in the batch.
+#include "fft_fast.h"
 #include "pass_util.h"
 
 namespace tdda {
@@ -512,6 +513,7 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
         case 2:
             zpass_generic<Real>(ls.pm[pr][PASS_Z][0], ls.jobs, t, gate, st);
             launch_fast_z(ls.pm[pr][PASS_Z][1], ls.jobs, tw, c64, gate, st);
+            if (c64) launch_fast_yz(ls.yz, ls.jobs, tw, gate, st);   // y, z and inverse y of the fused jobs
             break;
         default:
             xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
@@ -648,27 +650,34 @@ void plan_tiles(GridJob& J) {
     size_t unused = 0;
     J.mlx = quotient_magic(J.lx);
     J.mny = quotient_magic(J.ny);
-    J.fx = fast_tiles(J.lx, PASS_X, true, &J.bx32, &unused) ? 1 : 0;
+    // schedule variants (both variants of a length are specialised, so the
+    // fast / generic choice below does not depend on them)
+    for (int pr = 0; pr < 2; ++pr)
+        for (int pass = 0; pass < NPASS; ++pass) {
+            const int L = (pass == PASS_X || pass == PASS_XF) ? J.lx : pass == PASS_Y ? J.ly : J.lz;
+            J.var[pr][pass] = split_variants(pass, L) > 1 ? schedule_variant(pr, pass, L) : 0;
+        }
+    J.fx = fast_tiles(J.lx, PASS_X, true, J.var[0][PASS_X], &J.bx32, &unused) ? 1 : 0;
     if (J.fx) {
-        fast_tiles(J.lx, PASS_X, false, &J.bx64, &unused);
-        fast_tiles(J.lx, PASS_XF, true, &J.bxf32, &unused);
-        fast_tiles(J.lx, PASS_XF, false, &J.bxf64, &unused);
+        fast_tiles(J.lx, PASS_X, false, J.var[1][PASS_X], &J.bx64, &unused);
+        fast_tiles(J.lx, PASS_XF, true, J.var[0][PASS_XF], &J.bxf32, &unused);
+        fast_tiles(J.lx, PASS_XF, false, J.var[1][PASS_XF], &J.bxf64, &unused);
     } else {
         J.bx32 = std::max(1, Caps<float>::X / (3 * J.lx));
         J.bx64 = std::max(1, Caps<double>::X / (3 * J.lx));
         J.bxf32 = J.bx32;
         J.bxf64 = J.bx64;
     }
-    J.fy = fast_tiles(J.ly, PASS_Y, true, &J.by32, &unused) ? 1 : 0;
+    J.fy = fast_tiles(J.ly, PASS_Y, true, J.var[0][PASS_Y], &J.by32, &unused) ? 1 : 0;
     if (J.fy) {
-        fast_tiles(J.ly, PASS_Y, false, &J.by64, &unused);
+        fast_tiles(J.ly, PASS_Y, false, J.var[1][PASS_Y], &J.by64, &unused);
     } else {
         J.by32 = std::max(1, Caps<float>::Y / J.ly);
         J.by64 = std::max(1, Caps<double>::Y / J.ly);
     }
-    J.fz = fast_tiles(J.lz, PASS_Z, true, &J.bz32, &unused) ? 1 : 0;
+    J.fz = fast_tiles(J.lz, PASS_Z, true, J.var[0][PASS_Z], &J.bz32, &unused) ? 1 : 0;
     if (J.fz) {
-        fast_tiles(J.lz, PASS_Z, false, &J.bz64, &unused);
+        fast_tiles(J.lz, PASS_Z, false, J.var[1][PASS_Z], &J.bz64, &unused);
     } else {
         J.bz32 = std::max(1, Caps<float>::Z / J.lz);
         J.bz64 = std::max(1, Caps<double>::Z / J.lz);
@@ -683,6 +692,8 @@ void plan_tiles(GridJob& J) {
     J.ty64 = cdiv(3L * J.nz * J.lx, J.by64);
     J.tz64 = cdiv((long)J.ly * J.lx, J.bz64);
     J.tupd = cdiv(3L * J.ndip, (long)FFT_THREADS * UPD_EPT);
+    J.yzc = J.fx ? yz_class(J.ly, J.lz, J.ny, J.nz) : -1;
+    J.tr32 = 0;   // set per batch
 }
 
 size_t lds_bytes(const GridJob& J, int pass, bool c64) {
@@ -692,7 +703,7 @@ size_t lds_bytes(const GridJob& J, int pass, bool c64) {
     if (fast) {
         int per = 0;
         size_t bytes = 0;
-        fast_tiles(L, pass, c64, &per, &bytes);
+        fast_tiles(L, pass, c64, J.var[c64 ? 0 : 1][pass], &per, &bytes);
         return bytes;
     }
     if (pass == PASS_X) return (size_t)J.lx * ((3 * (c64 ? J.bx32 : J.bx64)) | 1) * cs;

@@ -29,7 +29,7 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    with_split_xy(J.lx, [&](auto s1, auto s2) {
+    with_split_pass<PASS_XF>(J.lx, J.var[sizeof(Real) == 4 ? 0 : 1][PASS_XF], [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         using X = XFast<L1, L2, fast_nr_x<Real>(L1, L2), C>;
         C* lds = reinterpret_cast<C*>(smem_xff);
@@ -50,6 +50,14 @@ k_xfwd_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
         const unsigned ny = J.ny, ly = J.ly;
         const unsigned S = (unsigned)J.nz * ly * L;
         const unsigned long long mny = J.mny;
+        if (sizeof(Real) == 4 && J.tr32) {   // [kx][3][nz][ny] for the fused y/z pass
+            const unsigned nzy = (unsigned)J.nz * ny, slab = 3u * nzy;
+            for (int t = threadIdx.x; t < X::TB; t += FFT_THREADS)
+                X::fwd_b(t, lds, nr, [&](int r, int c, int x, C v) {
+                    gstore<C>(g, x * slab + c * nzy + (unsigned)(q0 + r), v);
+                });
+            return;
+        }
         for (int t = threadIdx.x; t < X::TB; t += FFT_THREADS)
             X::fwd_b(t, lds, nr, [&](int r, int c, int x, C v) {
                 const unsigned q = q0 + r;
@@ -72,7 +80,7 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
     double acc[XI_NACC] = {};
-    with_split(J.lx, [&](auto s1, auto s2) {
+    with_split_pass<PASS_X>(J.lx, J.var[sizeof(Real) == 4 ? 0 : 1][PASS_X], [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         using X = XFast<L1, L2, fast_nr_x<Real>(L1, L2), C>;
         C* lds = reinterpret_cast<C*>(smem_xif);
@@ -85,13 +93,21 @@ k_xinv_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, con
         const unsigned S = (unsigned)J.nz * ly * L;
         const unsigned long long mny = J.mny;
         __syncthreads();
-        for (int t = threadIdx.x; t < X::TA; t += FFT_THREADS)
-            X::inv_a(t, lds, twl, nr, [&](int r, int c, int x) -> C {
-                const unsigned q = q0 + r;
-                const unsigned z = fdiv(q, mny);
-                const unsigned y = q - z * ny;
-                return gload<C>(g, c * S + (z * ly + y) * L + x);
-            });
+        if (sizeof(Real) == 4 && J.tr32) {   // [kx][3][nz][ny] after the fused y/z pass
+            const unsigned nzy = (unsigned)J.nz * ny, slab = 3u * nzy;
+            for (int t = threadIdx.x; t < X::TA; t += FFT_THREADS)
+                X::inv_a(t, lds, twl, nr, [&](int r, int c, int x) -> C {
+                    return gload<C>(g, x * slab + c * nzy + (unsigned)(q0 + r));
+                });
+        } else {
+            for (int t = threadIdx.x; t < X::TA; t += FFT_THREADS)
+                X::inv_a(t, lds, twl, nr, [&](int r, int c, int x) -> C {
+                    const unsigned q = q0 + r;
+                    const unsigned z = fdiv(q, mny);
+                    const unsigned y = q - z * ny;
+                    return gload<C>(g, c * S + (z * ly + y) * L + x);
+                });
+        }
         __syncthreads();
         const Real inv_m = (Real)(1.0 / ((double)J.lx * (double)J.ly * (double)J.lz));
         const C* al = Tile<Real>::al(J);
@@ -117,7 +133,7 @@ k_ypass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    with_split(J.ly, [&](auto s1, auto s2) {
+    with_split_pass<PASS_Y>(J.ly, J.var[sizeof(Real) == 4 ? 0 : 1][PASS_Y], [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_y<Real>(L1, L2);
         using Y = YFast<L1, L2, NP, DIR, C>;
@@ -164,7 +180,7 @@ k_zpass_fast(const GridJob* __restrict__ jobs, const int2* __restrict__ bmap, co
     const int2 jt = bmap[blockIdx.x];
     const GridJob& J = jobs[jt.x];
     if (!gate_open(J, gate)) return;
-    with_split_z(J.lz, [&](auto s1, auto s2) {
+    with_split_pass<PASS_Z>(J.lz, J.var[sizeof(Real) == 4 ? 0 : 1][PASS_Z], [&](auto s1, auto s2) {
         if constexpr (z_class_of<decltype(s2)>() != CLS) return;   // the other class's launch
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, L = L1 * L2;
         constexpr int NP = fast_np_z<Real>(L1, L2);
@@ -281,7 +297,7 @@ void launch_fast_z(const PassMap& pm, const GridJob* jobs, const TwiddlePools& t
 }
 
 // tile geometry of the specialised path (host)
-bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
+bool fast_tiles(int L, int pass, bool c64, int v, int* per_tile, size_t* lds_bytes) {
     auto tiles = [&](auto s1, auto s2) {
         constexpr int L1 = decltype(s1)::value, L2 = decltype(s2)::value, LL = L1 * L2;
         const size_t cs = c64 ? sizeof(float2) : sizeof(double2);
@@ -296,9 +312,10 @@ bool fast_tiles(int L, int pass, bool c64, int* per_tile, size_t* lds_bytes) {
             *lds_bytes = ((size_t)3 * *per_tile * LL + LL) * cs;
         }
     };
-    if (pass == PASS_Z) return with_split_z(L, tiles);
-    if (pass == PASS_XF) return with_split_xy(L, tiles);
-    return with_split(L, tiles);
+    if (pass == PASS_Z) return with_split_pass<PASS_Z>(L, v, tiles);
+    if (pass == PASS_XF) return with_split_pass<PASS_XF>(L, v, tiles);
+    if (pass == PASS_Y) return with_split_pass<PASS_Y>(L, v, tiles);
+    return with_split_pass<PASS_X>(L, v, tiles);
 }
 
 }  // namespace tdda

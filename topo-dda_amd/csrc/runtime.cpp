@@ -612,7 +612,9 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
                   hipStream_t st, bool interleave_shared) {
     hjobs.clear();
     std::vector<int2> m[2][NPASS][2];
+    std::vector<int2> myz[NYZ];
     std::vector<int2> mu;
+    const int fuse = fused_yz();
     size_t lds[2][NPASS][2] = {};
     int zcls[2] = {0, 0};                  // specialised Z length classes present
     size_t zlds[2][2] = {};
@@ -621,16 +623,34 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         TDDA_REQUIRE(!refined || s->refined, ERR_INVALID, "refined solve needs refined systems");
         GridJob J = s->job;
         J.ks = j < ks.size() ? ks[j] : nullptr;
+        // c64 matvec through the fused y/z pass: x frequencies paired with
+        // their mirrors (kx, lx - kx read the same octant records)
+        J.tr32 = J.yzc >= 0 && (fuse == FUSED_ALL || (fuse == FUSED_SCHEDULE && schedule_fused(J.ly, J.lz))) ? 1 : 0;
         hjobs.push_back(J);
         const int jj = (int)j;
+        if (J.tr32) {   // 0 and lx/2 are their own mirrors
+            std::vector<int2>& mz = myz[J.yzc];
+            mz.push_back(make_int2(jj, 0));
+            if (J.lx % 2 == 0) mz.push_back(make_int2(jj, J.lx / 2));
+            for (int a = 1; 2 * a < J.lx; ++a) {
+                mz.push_back(make_int2(jj, a));
+                mz.push_back(make_int2(jj, J.lx - a));
+            }
+        }
         for (int pr = 0; pr < 2; ++pr) {
             const bool c64 = pr == 0;
+            const bool fused = c64 && J.tr32;
             const int tx = c64 ? J.tx32 : J.tx64, ty = c64 ? J.ty32 : J.ty64;
             for (int t = 0; t < tx; ++t) m[pr][PASS_X][J.fx].push_back(make_int2(jj, t));
             const int txf = c64 ? J.txf32 : J.txf64;
             for (int t = 0; t < txf; ++t) m[pr][PASS_XF][J.fx].push_back(make_int2(jj, t));
             size_t& lxf = lds[pr][PASS_XF][J.fx];
             lxf = std::max(lxf, lds_bytes(J, PASS_XF, c64));
+            if (fused) {
+                size_t& lx = lds[pr][PASS_X][J.fx];
+                lx = std::max(lx, lds_bytes(J, PASS_X, c64));
+                continue;
+            }
             for (int t = 0; t < ty; ++t) m[pr][PASS_Y][J.fy].push_back(make_int2(jj, t));
             size_t& lx = lds[pr][PASS_X][J.fx];
             size_t& ly = lds[pr][PASS_Y][J.fy];
@@ -639,7 +659,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             ly = std::max(ly, lds_bytes(J, PASS_Y, c64));
             lz = std::max(lz, lds_bytes(J, PASS_Z, c64));
             if (J.fz) {
-                const int zc = z_class(J.lz);
+                const int zc = z_class(J.lz, J.var[pr][PASS_Z]);
                 zcls[pr] |= 1 << zc;
                 zlds[pr][zc] = std::max(zlds[pr][zc], lds_bytes(J, PASS_Z, c64));
             }
@@ -671,6 +691,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         }
         for (const auto& g : groups) {
             const GridJob& J0 = hjobs[g[0]];
+            if (pr == 0 && J0.tr32) continue;   // fused y/z pass (same shape for the whole group)
             const int T = pr == 0 ? J0.tz32 : J0.tz64;
             const int per = pr == 0 ? J0.bz32 : J0.bz64;
             const std::vector<int> order = z_tile_order(J0, per, T);
@@ -683,10 +704,12 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         }
         for (int k = 0; k < 2; ++k) xcd_permute(m[pr][PASS_Z][k]);
     }
+    for (auto& v : myz) xcd_permute(v);
     size_t total = mu.size();
     for (auto& a : m)
         for (auto& b : a)
             for (auto& v : b) total += v.size();
+    for (auto& v : myz) total += v.size();
     std::vector<int2>& all = hmaps_;   // kept alive: the upload is asynchronous
     all.clear();
     all.reserve(total);
@@ -699,6 +722,11 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             }
     const size_t off_u = all.size();
     all.insert(all.end(), mu.begin(), mu.end());
+    size_t off_yz[NYZ];   // after the c128 maps and the update map, which the compaction reads as one run
+    for (int c = 0; c < NYZ; ++c) {
+        off_yz[c] = all.size();
+        all.insert(all.end(), myz[c].begin(), myz[c].end());
+    }
     // stage layout: jobs, maps, reliable-update ranges (256-byte aligned)
     const int nj_all = (int)hjobs.size();
     auto al256 = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -731,6 +759,12 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     ls.map_upd = base + off_u;
     ls.n_upd = (int)mu.size();
     ls.refined = refined;
+    for (int c = 0; c < NYZ; ++c) {
+        PassMap& pm = ls.yz[c];
+        pm.map = base + off_yz[c];
+        pm.n = (int)myz[c].size();
+        pm.lds = pm.n ? yz_lds_bytes(c) : 0;
+    }
     // reliable-update compaction regions: the c128 maps, contiguous from
     // pm[1][X][0] in build order
     if (refined) {

@@ -87,6 +87,13 @@ def lib():
         L.tdda_server_idle.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tdda_set_graph_capture.argtypes = [ctypes.c_int]
+        L.tdda_set_fused_yz.argtypes = [ctypes.c_int]
+        L.tdda_schedule_text.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+        L.tdda_schedule_parse.argtypes = [ctypes.c_char_p]
+        L.tdda_schedule_load.argtypes = [ctypes.c_char_p]
+        L.tdda_schedule_save.argtypes = [ctypes.c_char_p]
+        L.tdda_tune.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
         L.tdda_device_bytes.restype = ctypes.c_long
         L.tdda_trim_caches.restype = ctypes.c_int
         _LIB = L
@@ -258,6 +265,66 @@ def set_graph_capture(on: bool):
     """Record batched iterations as HIP graphs and replay them (default), or
     issue the launches one by one."""
     check(lib().tdda_set_graph_capture(int(bool(on))))
+
+
+FUSED = {"off": 0, "schedule": 1, "all": 2}
+
+
+def set_fused_yz(mode):
+    """Which eligible boxes run the c64 y, z and inverse y passes as one
+    kernel per x frequency: "off", "schedule" (as the kernel schedule says;
+    the default) or "all"; applies to batches built afterwards."""
+    check(lib().tdda_set_fused_yz(FUSED[mode]))
+
+
+def _text(call):
+    n = ctypes.c_long(0)
+    check(call(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(call(buf, n.value + 1, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def schedule_text():
+    """The kernel schedule, one entry per line."""
+    return _text(lib().tdda_schedule_text)
+
+
+def schedule_set(text: str):
+    check(lib().tdda_schedule_parse(text.encode()))
+
+
+def schedule_load(path):
+    check(lib().tdda_schedule_load(str(path).encode()))
+
+
+def schedule_save(path):
+    check(lib().tdda_schedule_save(str(path).encode()))
+
+
+def schedule_clear():
+    check(lib().tdda_schedule_clear())
+
+
+def tune(boxes, count=4, reps=10):
+    """Autotune the kernel schedule for these (nx, ny, nz) boxes; returns the
+    decisions as dicts (pass timings in ms summed over the boxes)."""
+    b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 3))
+    arr = b.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    n = ctypes.c_long(0)
+    cap = 1 << 16
+    buf = ctypes.create_string_buffer(cap)
+    check(lib().tdda_tune(arr, b.shape[0], int(count), int(reps), buf, cap, ctypes.byref(n)))
+    out = []
+    for line in buf.value.decode().splitlines():
+        f = line.split()
+        if f[0] == "pass":
+            out.append({"kind": "pass", "precision": f[1], "pass": f[2], "L": int(f[3]), "ms": float(f[5]),
+                        "ms_alt": None if f[6] == "-" else float(f[6]), "chosen": int(f[8])})
+        else:
+            out.append({"kind": "yz", "Ly": int(f[1]), "Lz": int(f[2]), "ms_three": float(f[4]),
+                        "ms_fused": float(f[5]), "chosen": int(f[7])})
+    return out
 
 
 def time_passes(problems, c64=True, reps=20):
