@@ -1,0 +1,85 @@
+"""GPU tests of the kernel schedule and its autotuner.
+
+The tuner must record a decision for every pass length of the boxes it is
+given (variant 1 only where the two split tables differ) and for every fused
+y/z class; whatever it records, the transforms stay the same operator: c64
+applies under the tuned schedule and under every variant 1 forced match the
+FP64 apply to 2e-6, and the FP64 apply matches the oracle to 1e-11."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _len(n):
+    L = max(2, 2 * n - 1)
+    while True:
+        m = L
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return L
+        L += 1
+
+
+BOXES = [(16, 16, 16), (48, 48, 24), (64, 40, 32)]
+
+
+def test_tuner_decisions_and_results():
+    from dda_model import native
+    import dda_oracle as O
+    _log("[gpu] tune " + str(BOXES))
+    native.schedule_clear()
+    try:
+        rep = native.tune(BOXES, count=2, reps=3)
+        passes = {(d["precision"], d["pass"], d["L"]): d for d in rep if d["kind"] == "pass"}
+        fused = {(d["Ly"], d["Lz"]): d for d in rep if d["kind"] == "yz"}
+        for prec in ("c64", "c128"):
+            for nx, ny, nz in BOXES:
+                for p, L in (("xfwd", _len(nx)), ("xinv", _len(nx)), ("y", _len(ny)), ("z", _len(nz))):
+                    d = passes[(prec, p, L)]
+                    assert d["ms"] > 0
+                    has_alt = (p in ("xfwd", "xinv", "y") and L in (80, 96, 128)) or (p == "z" and L in (40, 48, 64))
+                    assert (d["ms_alt"] is not None) == has_alt, (prec, p, L)
+                    assert d["chosen"] == (1 if has_alt and d["ms_alt"] < d["ms"] else 0)
+        # (16,16,16) -> (32, 32), (48,48,24) -> (96, 48); (64,40,32) has Ly = 80: no fused class
+        assert set(fused) == {(32, 32), (96, 48)}
+        text = native.schedule_text()
+        for (prec, p, L), d in passes.items():
+            if d["ms_alt"] is not None:
+                assert f"var {prec} {p} {L} {d['chosen']}" in text
+        _log("[gpu] tuned schedule:\n" + text)
+        rng = np.random.default_rng(31)
+        reqs = [native.Request(shape=b, d=15.0, lam=500.0 + 40 * i, rho=rng.uniform(0, 1, int(np.prod(b))))
+                for i, b in enumerate(BOXES)]
+        probs = [native.Problem(r, precision="fp32_refined") for r in reqs]
+        xs = [rng.standard_normal(3 * p.n) + 1j * rng.standard_normal(3 * p.n) for p in probs]
+        y64 = native.apply(probs, xs, c64=False)
+        for a, b in zip(native.apply(probs, xs, c64=True), y64):
+            assert _rel(a, b) < 2e-6
+        # every variant 1 forced (new systems pick it up at their tile planning)
+        alt = "".join(f"var {prec} {p} {L} 1\n" for (prec, p, L), d in passes.items() if d["ms_alt"] is not None)
+        alt += "yz 32 32 1\nyz 96 48 1\n"
+        native.schedule_set(alt)
+        probs2 = [native.Problem(r, precision="fp32_refined") for r in reqs]
+        y64b = native.apply(probs2, xs, c64=False)
+        y32b = native.apply(probs2, xs, c64=True)
+        for b, a64, a32, ref in zip(BOXES, y64b, y32b, y64):
+            _log(f"[gpu] {b} all variants 1: fp64 vs fp64 default {_rel(a64, ref):.2e}, c64 vs fp64 {_rel(a32, ref):.2e}")
+            assert _rel(a64, ref) < 1e-13 and _rel(a32, ref) < 2e-6
+        box = O.OracleFullBox(BOXES[1], 15.0, reqs[1].lam, reqs[1].rho, workers=8)
+        assert _rel(y64b[1], box.apply(xs[1])) < 1e-11
+    finally:
+        native.schedule_clear()
