@@ -374,6 +374,8 @@ def main():
     from dda_model import native
 
     if a.roofline_only:
+        if not a.no_tune:
+            native.tune(SHAPES, count=4, reps=10)
         ms5, ms10, _ = roofline_probe(native, 30)
         print(json.dumps({"pass_ms": [float(x) for x in ms5], "iteration_pass_ms": [float(x) for x in ms10],
                           "copy_bytes": 2 * (256 << 20),

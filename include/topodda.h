@@ -153,6 +153,12 @@ int tdda_set_graph_capture(int on);
 #define TDDA_FUSED_SCHEDULE 1
 #define TDDA_FUSED_ALL 2
 int tdda_set_fused_yz(int mode);
+/* c64 transforms of boxes with Lx = Ly in {32, 48, 64, 96, 128} can run the
+ * x and y transforms of each z-plane as one kernel per direction (the plane
+ * stays in LDS between them; on by default).  on = 0: the separate X and Y
+ * passes.  Applies to batches built afterwards; results agree to c64
+ * rounding.  No reference counterpart. */
+int tdda_set_plane_passes(int on);
 
 /* Kernel schedule: the split variant of each length-specialised pass per
  * transform length and precision, and the fused y/z choice per (Ly, Lz).

@@ -29,3 +29,22 @@ def test_fast_passes_match_naive_dft(tmp_path):
     assert "FAIL" not in r.stdout
     worst = float(r.stdout.strip().splitlines()[-1].split()[-1])
     assert worst < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+def test_plane_passes_match_naive_2d_dft(tmp_path):
+    """The plane transforms (fft_plane.h) of every class, forward and inverse,
+    c64 and c128, against naive 2-D DFTs with pruned rows and columns; each
+    output written once, no read of an unwritten LDS entry, no index outside
+    the LDS region (AddressSanitizer)."""
+    exe = tmp_path / "fft_plane_check"
+    src = os.path.join(ROOT, "tests", "native", "fft_plane_check.hip")
+    inc = os.path.join(ROOT, "topo-dda_amd", "csrc")
+    cmd = [HIPCC, "-O1", "-std=c++17", "-I", inc, "-Xarch_host", "-fsanitize=address", "-o", str(exe), src]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=600)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    last = r.stdout.strip().splitlines()[-1].split()
+    assert last[:2] == ["fails", "0"] and float(last[-1]) < 1e-6

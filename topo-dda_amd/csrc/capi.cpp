@@ -57,6 +57,10 @@ int tdda_set_fused_yz(int mode) {
     return guarded([&] { tdda::set_fused_yz(mode); });
 }
 
+int tdda_set_plane_passes(int on) {
+    return guarded([&] { tdda::set_plane_passes(on != 0); });
+}
+
 static void copy_text(const std::string& t, char* buf, long cap, long* len) {
     if (len) *len = (long)t.size();
     if (buf && cap > 0) {

@@ -81,6 +81,11 @@ struct GridJob {
     // in a batch that runs this job through it (the X passes then keep the
     // work grid as [kx][3][nz][ny])
     int yzc, tr32;
+    // plane passes of the c64 matvec (X and Y transforms of a z-plane in one
+    // kernel per direction): the class of (Lx, Ly, Nx, Ny) (-1: none); xy32 = 1
+    // in a batch that runs this job through them (its X inverse partials are
+    // then per plane: tx32 = nz in that batch's copy)
+    int xyc, xy32;
     unsigned long long mlx, mny;   // ceil(2^32 / lx), ceil(2^32 / ny): fast quotients
     const int* box_map;        // [nz][ny][nx] -> dipole or -1; null: identity
     // operator data

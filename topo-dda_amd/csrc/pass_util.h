@@ -80,10 +80,12 @@ __device__ inline bool ru_idle(int gate, const int* ru_jobs) {
 
 // Deterministic block reduction of NV doubles; every thread gets the sums.
 // Butterfly over the 64 lanes of each wave (a fixed pattern, so the result
-// is reproducible), then the FFT_THREADS/64 wave sums in wave order.
-template <int NV>
+// is reproducible), then the NT/64 wave sums in wave order (NT: threads of
+// the block).
+template <int NV, int NT = FFT_THREADS>
 __device__ inline void block_sum(double (&v)[NV]) {
-    constexpr int NW = FFT_THREADS / 64;
+    static_assert(NT % 64 == 0, "whole waves");
+    constexpr int NW = NT / 64;
     __shared__ double red[NV][NW];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {

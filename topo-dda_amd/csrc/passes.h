@@ -49,6 +49,8 @@ constexpr int RU_REGIONS = 2 * NPASS;
 // Length classes of the fused y/z pass (c64): Ly = 16 LY1 with LY1 in
 // {2, 3, 4, 6, 8}, Lz in {32, 48, 64}.
 constexpr int NYZ = 15;
+// Classes of the plane passes (fft_plane.h: L in {32, 48, 64, 96, 128})
+constexpr int NXY_CLASSES = 5;
 
 // A launch over a batch: device array of jobs and, per transform precision
 // and pass, the blocks of the generic kernels and of the length-specialised
@@ -75,6 +77,10 @@ struct LaunchSet {
     // fused y/z pass of the c64 matvec, one map per length class; its jobs
     // are in no c64 Y or Z map
     PassMap yz[NYZ];
+    // plane passes of the c64 matvec, one (job, z) map per class; their jobs
+    // are in no c64 X, X-forward or Y map
+    PassMap xy[NXY_CLASSES];
+    int n_xy = 0;   // blocks over all classes
 };
 
 // Twiddle pools (device), indexed by FftPlan::tw_off
@@ -133,6 +139,19 @@ void launch_fast_yz(const PassMap* yz, const GridJob* jobs, const TwiddlePools& 
 enum FusedMode : int { FUSED_OFF = 0, FUSED_SCHEDULE = 1, FUSED_ALL = 2 };
 void set_fused_yz(int mode);
 int fused_yz();
+// Plane passes (passes_xy.hip): X forward + Y forward and Y inverse + X
+// inverse of c64 matvecs, one workgroup per (job, z-plane) of every
+// non-empty class map; modes XF_PLAIN/XF_P/XF_T and XI_PLAIN/XI_V/XI_S.
+void launch_plane_fwd(const PassMap* xy, const GridJob* jobs, const TwiddlePools& tw, int mode, int gate,
+                      hipStream_t st);
+void launch_plane_inv(const PassMap* xy, const GridJob* jobs, const TwiddlePools& tw, int mode, int gate,
+                      hipStream_t st);
+size_t plane_lds_bytes(int cls, bool inverse);
+int plane_class_of(int lx, int ly, int nx, int ny);
+// Whether batches built from now on run eligible c64 jobs through the plane
+// passes (on by default).
+void set_plane_passes(bool on);
+bool plane_passes();
 // Rows (X) or pencils (Y, Z) per tile and LDS bytes of the specialised path
 // for length L; false if L has no specialised split.
 bool fast_tiles(int L, int pass, bool c64, int v, int* per_tile, size_t* lds_bytes);

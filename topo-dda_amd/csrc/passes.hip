@@ -504,6 +504,7 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
         case 0:
             xfwd_generic<Real, KReal>(ls.pm[pr][PASS_XF][0], ls.jobs, t, xf, gate, st);
             launch_fast_xfwd(ls.pm[pr][PASS_XF][1], ls.jobs, tw, c64, k32, xf, gate, st);
+            if (c64) launch_plane_fwd(ls.xy, ls.jobs, tw, xf, gate, st);   // x and y of the plane jobs
             break;
         case 1:
         case 3:
@@ -518,6 +519,7 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
         default:
             xinv_generic<Real, KReal>(ls.pm[pr][PASS_X][0], ls.jobs, t, xi, gate, st);
             launch_fast_xinv(ls.pm[pr][PASS_X][1], ls.jobs, tw, c64, k32, xi, gate, st);
+            if (c64) launch_plane_inv(ls.xy, ls.jobs, tw, xi, gate, st);   // y and x of the plane jobs
             if (xi != XI_PLAIN && ls.njobs > 0) {
                 const dim3 grid(ls.njobs), blk(FFT_THREADS);
                 switch (xi) {
@@ -534,7 +536,7 @@ static void pass_impl(int k, const LaunchSet& ls, const TwiddlePools& tw, int xf
 template <typename Real, typename KReal>
 static void matvec_impl(const LaunchSet& ls, const TwiddlePools& tw, int xf, int xi, int gate, hipStream_t st) {
     const int pr = sizeof(Real) == 4 ? 0 : 1;
-    if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n == 0) return;
+    if (ls.pm[pr][PASS_X][0].n + ls.pm[pr][PASS_X][1].n + (pr == 0 ? ls.n_xy : 0) == 0) return;
     if (gate == GATE_RU && pr == 1 && ls.ru_map != nullptr && ls.ru_ranges != nullptr && ls.ru_off[RU_REGIONS] > 0) {
         // only the tiles of jobs that wait for their reliable update (the
         // counts are cleared by a kernel, not a memset: this sequence is
@@ -694,6 +696,8 @@ void plan_tiles(GridJob& J) {
     J.tupd = cdiv(3L * J.ndip, (long)FFT_THREADS * UPD_EPT);
     J.yzc = J.fx ? yz_class(J.ly, J.lz, J.ny, J.nz) : -1;
     J.tr32 = 0;   // set per batch
+    J.xyc = plane_class_of(J.lx, J.ly, J.nx, J.ny);
+    J.xy32 = 0;   // set per batch
 }
 
 size_t lds_bytes(const GridJob& J, int pass, bool c64) {

@@ -424,7 +424,8 @@ System::System(std::shared_ptr<Operator> op_, int ndip_, const int* box_map_host
     job.hx = o.hx; job.hy = o.hy; job.hz = o.hz;
     job.px = o.px; job.py = o.py; job.pz = o.pz;
     plan_tiles(job);
-    const long nparts = std::max({(long)job.tupd, (long)job.tx32, (long)job.tx64});
+    // (c64 plane passes write one X inverse partial per z-plane)
+    const long nparts = std::max({(long)job.tupd, (long)job.tx32, (long)job.tx64, (long)job.nz});
     // reliable updates run the FP64 X forward pass: its tiles hold the
     // field-error partials
     const long nxf = refined ? (long)job.txf64 * XF_NACC : 0;
@@ -613,8 +614,10 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     hjobs.clear();
     std::vector<int2> m[2][NPASS][2];
     std::vector<int2> myz[NYZ];
+    std::vector<int2> mxy[NXY_CLASSES];
     std::vector<int2> mu;
     const int fuse = fused_yz();
+    const bool planes = plane_passes();
     size_t lds[2][NPASS][2] = {};
     int zcls[2] = {0, 0};                  // specialised Z length classes present
     size_t zlds[2][2] = {};
@@ -626,8 +629,14 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         // c64 matvec through the fused y/z pass: x frequencies paired with
         // their mirrors (kx, lx - kx read the same octant records)
         J.tr32 = J.yzc >= 0 && (fuse == FUSED_ALL || (fuse == FUSED_SCHEDULE && schedule_fused(J.ly, J.lz))) ? 1 : 0;
+        // c64 X and Y transforms through the plane passes (one block per
+        // z-plane, whose X inverse partials are per plane)
+        J.xy32 = planes && !J.tr32 && J.xyc >= 0 ? 1 : 0;
+        if (J.xy32) J.tx32 = J.nz;
         hjobs.push_back(J);
         const int jj = (int)j;
+        if (J.xy32)
+            for (int z = 0; z < J.nz; ++z) mxy[J.xyc].push_back(make_int2(jj, z));
         if (J.tr32) {   // 0 and lx/2 are their own mirrors
             std::vector<int2>& mz = myz[J.yzc];
             mz.push_back(make_int2(jj, 0));
@@ -640,6 +649,16 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         for (int pr = 0; pr < 2; ++pr) {
             const bool c64 = pr == 0;
             const bool fused = c64 && J.tr32;
+            if (c64 && J.xy32) {   // only the Z pass runs through the pass maps
+                size_t& lz = lds[pr][PASS_Z][J.fz];
+                lz = std::max(lz, lds_bytes(J, PASS_Z, c64));
+                if (J.fz) {
+                    const int zc = z_class(J.lz, J.var[pr][PASS_Z]);
+                    zcls[pr] |= 1 << zc;
+                    zlds[pr][zc] = std::max(zlds[pr][zc], lds_bytes(J, PASS_Z, c64));
+                }
+                continue;
+            }
             const int tx = c64 ? J.tx32 : J.tx64, ty = c64 ? J.ty32 : J.ty64;
             for (int t = 0; t < tx; ++t) m[pr][PASS_X][J.fx].push_back(make_int2(jj, t));
             const int txf = c64 ? J.txf32 : J.txf64;
@@ -705,11 +724,13 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         for (int k = 0; k < 2; ++k) xcd_permute(m[pr][PASS_Z][k]);
     }
     for (auto& v : myz) xcd_permute(v);
+    for (auto& v : mxy) xcd_permute(v);
     size_t total = mu.size();
     for (auto& a : m)
         for (auto& b : a)
             for (auto& v : b) total += v.size();
     for (auto& v : myz) total += v.size();
+    for (auto& v : mxy) total += v.size();
     std::vector<int2>& all = hmaps_;   // kept alive: the upload is asynchronous
     all.clear();
     all.reserve(total);
@@ -726,6 +747,11 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     for (int c = 0; c < NYZ; ++c) {
         off_yz[c] = all.size();
         all.insert(all.end(), myz[c].begin(), myz[c].end());
+    }
+    size_t off_xy[NXY_CLASSES];
+    for (int c = 0; c < NXY_CLASSES; ++c) {
+        off_xy[c] = all.size();
+        all.insert(all.end(), mxy[c].begin(), mxy[c].end());
     }
     // stage layout: jobs, maps, reliable-update ranges (256-byte aligned)
     const int nj_all = (int)hjobs.size();
@@ -764,6 +790,13 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         pm.map = base + off_yz[c];
         pm.n = (int)myz[c].size();
         pm.lds = pm.n ? yz_lds_bytes(c) : 0;
+    }
+    ls.n_xy = 0;
+    for (int c = 0; c < NXY_CLASSES; ++c) {
+        PassMap& pm = ls.xy[c];
+        pm.map = base + off_xy[c];
+        pm.n = (int)mxy[c].size();
+        ls.n_xy += pm.n;
     }
     // reliable-update compaction regions: the c128 maps, contiguous from
     // pm[1][X][0] in build order

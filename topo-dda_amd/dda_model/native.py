@@ -88,6 +88,7 @@ def lib():
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tdda_set_graph_capture.argtypes = [ctypes.c_int]
         L.tdda_set_fused_yz.argtypes = [ctypes.c_int]
+        L.tdda_set_plane_passes.argtypes = [ctypes.c_int]
         L.tdda_schedule_text.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
         L.tdda_schedule_parse.argtypes = [ctypes.c_char_p]
         L.tdda_schedule_load.argtypes = [ctypes.c_char_p]
@@ -275,6 +276,13 @@ def set_fused_yz(mode):
     kernel per x frequency: "off", "schedule" (as the kernel schedule says;
     the default) or "all"; applies to batches built afterwards."""
     check(lib().tdda_set_fused_yz(FUSED[mode]))
+
+
+def set_plane_passes(on: bool):
+    """Run the c64 x and y transforms of eligible boxes (Lx = Ly in {32, 48,
+    64, 96, 128}) as one plane kernel per direction (default) or as the
+    separate X and Y passes; applies to batches built afterwards."""
+    check(lib().tdda_set_plane_passes(int(bool(on))))
 
 
 def _text(call):
