@@ -197,18 +197,23 @@ def lattice_sum_rates(native, tables, threads):
             "gpu": round(gpu_rate, 1), "gpu_setup_ms": round(setup_ms, 3), "gpu_images": 50}
 
 
-PROBE_PER_SHAPE = 8
-TRAFFIC_FILE = "matvec_probe40_pmc.md"
+PROBE_PER_SHAPE = 48
+TRAFFIC_FILE = "matvec_probe240_pmc.md"
+# launch groups of the batched c64 matvec (pass slots of time_passes): the
+# x+y forward transform (plane passes; separate X and Y passes for boxes
+# outside the plane classes), the z pass with the Green's multiply, and the
+# inverse y+x transform with the al x epilogue
+GROUPS = [("xy_fwd", (0, 1)), ("z", (2,)), ("yx_inv", (3, 4))]
 
 
 def matvec_traffic():
-    """HBM bytes per pass launch (X fwd, Y fwd, Z, Y inv, X inv) on the probe
-    batch, from the committed rocprofv3 counter summary (counters cannot be
-    read inside this process); None if absent."""
+    """HBM bytes per launch group (xy_fwd, z, yx_inv) of one batched matvec
+    on the probe batch, from the committed rocprofv3 counter summary
+    (counters cannot be read inside this process); None if absent."""
     try:
         with open(os.path.join(ROOT, "profiles", TRAFFIC_FILE)) as f:
             for line in f:
-                if line.startswith("pass_traffic_bytes:"):
+                if line.startswith("group_traffic_bytes:"):
                     return [int(x) for x in line.split(":")[1].split(",")]
     except OSError:
         pass
@@ -219,15 +224,17 @@ def probe_shapes():
     return [s for _ in range(PROBE_PER_SHAPE) for s in SHAPES]
 
 
-def roofline_probe(native, reps):
-    """One batched launch per pass over a serving-like batch (8 problems of
-    every shape, distinct wavelengths so no two share a Green's spectrum, c64),
-    and a device copy of known size for calibrating byte counters."""
+def roofline_probe(native, reps, iteration=True):
+    """One batched matvec (one launch per pass and class) over a serving-like
+    batch (48 problems of every shape, about what the server's 256 slots hold, distinct
+    wavelengths so no two share a Green's spectrum, c64), optionally the ten
+    launches of one refined BiCGSTAB step, and a device copy of known size for
+    calibrating byte counters."""
     import torch
-    probs = [native.Problem(native.Request(shape=s, d=15.0, lam=500.0 + 3.0 * i, rho=np.full(int(np.prod(s)), 0.5)),
+    probs = [native.Problem(native.Request(shape=s, d=15.0, lam=500.0 + 1.0 * i, rho=np.full(int(np.prod(s)), 0.5)),
                             precision="fp32_refined") for i, s in enumerate(probe_shapes())]
     ms5 = native.time_passes(probs, c64=True, reps=reps)
-    ms10 = native.time_iteration(probs, reps=reps)
+    ms10 = native.time_iteration(probs, reps=reps) if iteration else None
     src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")   # 256 MiB
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -242,35 +249,37 @@ def roofline_record(ms5):
     (c64), n = 3 Nx Ny Nz unknowns and M = Lx Ly Lz the padded grid,
     B_mv = s (2n + 12 M) + 6 s M: x in and y out, one read and one write of
     the 3-component grid per transform direction, and the 6-component Green's
-    spectrum once.  `achieved` is the batch's sum of B_mv over the five pass
-    launches' summed duration (HIP events on the launch stream).  The passes
-    move less than B_mv (pruned zero padding, octant spectrum), so the
-    measured HBM bytes are reported beside it as `traffic` and `actual_GBs`."""
+    spectrum once.  `achieved` is the batch's sum of B_mv over the matvec's
+    summed launch durations (HIP events on the launch stream).  The kernels
+    move less than B_mv (pruned zero padding, the plane kept in LDS between
+    the x and y transforms, octant spectrum), so the measured HBM bytes are
+    reported beside it as `traffic` and `actual_GBs`."""
     Ms = [fft_len(s[0]) * fft_len(s[1]) * fft_len(s[2]) for s in probe_shapes()]
     ns = [3 * int(np.prod(s)) for s in probe_shapes()]
     b_mv = sum(8 * (2 * n + 12 * M) + 8 * 6 * M for n, M in zip(ns, Ms))
     t = float(sum(ms5)) * 1e-3
     pt = matvec_traffic()
-    names = ["x_fwd", "y_fwd", "z", "y_inv", "x_inv"]
-    passes = {}
-    for k, nm in enumerate(names):
-        e = {"ms": round(float(ms5[k]), 5)}
+    groups = {}
+    for k, (nm, slots) in enumerate(GROUPS):
+        ms = float(sum(ms5[i] for i in slots))
+        e = {"ms": round(ms, 5)}
         if pt:
             e["bytes"] = pt[k]
-            e["GBs"] = round(pt[k] / (ms5[k] * 1e-3) / 1e9, 1)
-        passes[nm] = e
+            e["GBs"] = round(pt[k] / (ms * 1e-3) / 1e9, 1)
+        groups[nm] = e
     traffic = sum(pt) if pt else None
     return {"bound": "hbm",
-            "kernel": f"batched matvec y = A x + al x, c64: k_xfwd_fast, k_ypass_fast<-1>, k_zpass_fast, "
-                      f"k_ypass_fast<+1>, k_xinv_fast, one launch each over a {len(Ms)}-request batch "
-                      f"({PROBE_PER_SHAPE} per box shape)",
+            "kernel": f"batched c64 matvec y = A x + al x over a {len(Ms)}-request batch ({PROBE_PER_SHAPE} per box "
+                      f"shape): k_plane_fwd (x and y transforms of each z-plane), k_zpass_fast (z transform, Green's "
+                      f"multiply, inverse z), k_plane_inv (inverse y and x, al x epilogue), one launch per length class",
             "achieved": round(b_mv / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b_mv / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes": b_mv, "matvec_ms": round(t * 1e3, 5),
             "actual_GBs": round(traffic / t / 1e9, 1) if traffic else None,
             "actual_frac": round(traffic / t / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-            "traffic_source": f"profiles/{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, same batch)",
-            "passes": passes}
+            "traffic_source": f"profiles/{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE and WRITE_SIZE, each scaled by "
+                              f"its reading for a 256 MiB device copy, same batch)",
+            "groups": groups}
 
 
 C4_SHAPE = (128, 128, 64)
@@ -353,8 +362,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--fp64-requests", type=int, default=96)
-    ap.add_argument("--slots", type=int, default=128)
-    ap.add_argument("--chunk", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=2)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the 128x128x64 multi-RHS record")
@@ -374,11 +383,12 @@ def main():
     from dda_model import native
 
     if a.roofline_only:
+        # the matvec probe alone (after the tuner, as in the full run), so that
+        # a profile's last 30 launches of each kernel are the probe's
         if not a.no_tune:
-            native.tune(SHAPES, count=4, reps=10)
-        ms5, ms10, _ = roofline_probe(native, 30)
-        print(json.dumps({"pass_ms": [float(x) for x in ms5], "iteration_pass_ms": [float(x) for x in ms10],
-                          "copy_bytes": 2 * (256 << 20),
+            native.tune(SHAPES, count=16, reps=10)
+        ms5, _, _ = roofline_probe(native, 30, iteration=False)
+        print(json.dumps({"pass_ms": [float(x) for x in ms5], "copy_bytes": 2 * (256 << 20),
                           "roofline": roofline_record(ms5)}), flush=True)
         return
 
@@ -392,10 +402,11 @@ def main():
     tune_rec = {"tuned": False}
     if not a.no_tune:
         t0 = time.perf_counter()
-        decisions = native.tune(SHAPES, count=4, reps=10)
+        decisions = native.tune(SHAPES, count=16, reps=10)
         tune_rec = {"tuned": True, "seconds": round(time.perf_counter() - t0, 2),
                     "variant_1_chosen": sum(1 for d in decisions if d["kind"] == "pass" and d["chosen"] == 1),
                     "fused_yz_chosen": sum(1 for d in decisions if d["kind"] == "yz" and d["chosen"] == 1),
+                    "plane_chosen": [d["L"] for d in decisions if d["kind"] == "xy" and d["chosen"] == 1],
                     "decisions": len(decisions)}
         print(f"# tuned schedule in {tune_rec['seconds']} s", file=sys.stderr, flush=True)
 

@@ -161,9 +161,11 @@ int tdda_set_fused_yz(int mode);
 int tdda_set_plane_passes(int on);
 
 /* Kernel schedule: the split variant of each length-specialised pass per
- * transform length and precision, and the fused y/z choice per (Ly, Lz).
- * Text, one entry per line: "var <c64|c128> <xinv|y|z|xfwd> <L> <variant>",
- * "yz <Ly> <Lz> <0|1>"; absent entries mean variant 0 and three passes.  It
+ * transform length and precision, the fused y/z choice per (Ly, Lz) and the
+ * plane-pass choice per plane length L (plane passes take precedence over the
+ * fused y/z pass).  Text, one entry per line: "var <c64|c128>
+ * <xinv|y|z|xfwd> <L> <variant>", "yz <Ly> <Lz> <0|1>", "xy <L> <0|1>";
+ * absent entries mean variant 0, no fused y/z pass and plane passes on.  It
  * is read when systems are made (and batches built), and loaded at start-up
  * from the file named by the environment variable TDDA_SCHEDULE.  Text
  * outputs are NUL-terminated and cut to cap bytes; *len receives the full
@@ -176,10 +178,13 @@ int tdda_schedule_save(const char* path);
 int tdda_schedule_clear(void);
 /* Autotuner: for count LDR problems of each of the n boxes (nx, ny, nz
  * triples), times every split variant of every pass of the c64 and c128
- * matvecs (reps batched launches each) and the c64 matvec with the fused y/z
- * pass against three passes, records the faster choices in the schedule, and
- * writes one report line per decision ("pass <prec> <pass> <L> ms <v0> <v1|->
- * chosen <v>", "yz <Ly> <Lz> ms <three> <fused> chosen <0|1>").  Run it while
+ * matvecs (reps batched launches each), the c64 matvec with the fused y/z
+ * pass against three passes, and per plane length the c64 matvec with the
+ * plane passes against those choices; records the faster choices in the
+ * schedule, and writes one report line per decision ("pass <prec> <pass> <L>
+ * ms <v0> <v1|-> chosen <v>", "yz <Ly> <Lz> ms <three> <fused> chosen <0|1>",
+ * "xy <L> ms <off> <on> chosen <0|1>").  A choice
+ * leaves the default only when the alternative is at least 3 % faster.  Run it while
  * no other work is being set up: the variants it tries apply to every system
  * made meanwhile. */
 int tdda_tune(const int* boxes, int n, int count, int reps, char* report, long cap, long* len);
@@ -261,7 +266,9 @@ int tdda_problems_apply(tdda_problem** ps, int n, int c64, const double* x, doub
 int tdda_problem_apply_noalloc(tdda_problem* p, const double* x, double* y);
 /* Average ms per launch of the five matvec passes over all n problems in one
  * batched launch per pass, c64 (precision TDDA_PREC_FP32_REFINED problems) or
- * c128 transforms (c64 != 0 selects c64); HIP events on the library stream. */
+ * c128 transforms (c64 != 0 selects c64); HIP events on the library stream.
+ * Jobs on the plane passes count their x+y forward kernel in slot 0 and their
+ * inverse y+x kernel in slot 4; a slot with no launch reads 0. */
 int tdda_problems_time_passes(tdda_problem** ps, int n, int c64, int reps, double* ms5);
 /* Average ms per launch of the ten passes of one refined BiCGSTAB step over
  * all n problems (TDDA_PREC_FP32_REFINED): X fwd with the p update, Y, Z, Y,

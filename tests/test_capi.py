@@ -96,9 +96,10 @@ def test_schedule_text_format_and_variants(tmp_path):
     from dda_model import native
     native.schedule_clear()
     try:
-        native.schedule_set("# tuned\nvar c64 z 64 1\nvar c128 xfwd 128 1  # trailing comment\nyz 128 64 1\n")
+        native.schedule_set("# tuned\nvar c64 z 64 1\nvar c128 xfwd 128 1  # trailing comment\nyz 128 64 1\n"
+                            "xy 96 0\n")
         t = native.schedule_text()
-        assert "var c64 z 64 1" in t and "var c128 xfwd 128 1" in t and "yz 128 64 1" in t
+        assert "var c64 z 64 1" in t and "var c128 xfwd 128 1" in t and "yz 128 64 1" in t and "xy 96 0" in t
         path = tmp_path / "schedule.txt"
         native.schedule_save(path)
         native.schedule_clear()
@@ -108,7 +109,8 @@ def test_schedule_text_format_and_variants(tmp_path):
         for bad in ["var c64 z 32 1",          # 32: both tables hold 8 x 4
                     "var c64 z 64 2",          # no variant 2
                     "var c64 y 100 1",         # no split at all
-                    "var c32 z 64 1", "var c64 w 64 1", "yz 128 64 3", "yz 128", "tune 1 2 3"]:
+                    "var c32 z 64 1", "var c64 w 64 1", "yz 128 64 3", "yz 128", "tune 1 2 3",
+                    "xy 96 2", "xy 96", "xy 0 1"]:
             with pytest.raises(RuntimeError):
                 native.schedule_set(bad)
             assert native.schedule_text() == t, bad

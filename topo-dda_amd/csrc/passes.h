@@ -114,6 +114,7 @@ int* ru_job_counter();
 // fused y/z pass is enabled for (Ly, Lz) (off when none is recorded).
 int schedule_variant(int pr, int pass, int L);
 bool schedule_fused(int ly, int lz);
+bool schedule_plane(int L);
 // Host-side tile geometry for a job (fills the b*/t*/f*/var fields).
 void plan_tiles(GridJob& J);
 // Dynamic LDS bytes of the job's tile in a pass (generic or specialised).

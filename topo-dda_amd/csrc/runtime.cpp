@@ -628,11 +628,15 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
         J.ks = j < ks.size() ? ks[j] : nullptr;
         // c64 matvec through the fused y/z pass: x frequencies paired with
         // their mirrors (kx, lx - kx read the same octant records)
-        J.tr32 = J.yzc >= 0 && (fuse == FUSED_ALL || (fuse == FUSED_SCHEDULE && schedule_fused(J.ly, J.lz))) ? 1 : 0;
         // c64 X and Y transforms through the plane passes (one block per
-        // z-plane, whose X inverse partials are per plane)
-        J.xy32 = planes && !J.tr32 && J.xyc >= 0 ? 1 : 0;
+        // z-plane, whose X inverse partials are per plane); they take
+        // precedence over the fused y/z pass
+        J.xy32 = planes && J.xyc >= 0 && schedule_plane(J.lx) ? 1 : 0;
         if (J.xy32) J.tx32 = J.nz;
+        J.tr32 = !J.xy32 && J.yzc >= 0 &&
+                         (fuse == FUSED_ALL || (fuse == FUSED_SCHEDULE && schedule_fused(J.ly, J.lz)))
+                     ? 1
+                     : 0;
         hjobs.push_back(J);
         const int jj = (int)j;
         if (J.xy32)

@@ -295,7 +295,8 @@ void apply_operator32(System& s, const float2* x, float2* y, hipStream_t st);
 void apply_batch(const std::vector<System*>& v, bool c64, const double2* x, double2* y, hipStream_t st);
 
 // Average ms per launch of the five passes (X fwd, Y fwd, Z, Y inv, X inv)
-// of y = A x + al x over `reps` repetitions, c64 or c128 transforms.
+// of y = A x + al x over `reps` repetitions, c64 or c128 transforms (plane
+// jobs: x+y forward in slot 0, inverse y+x in slot 4; empty slots read 0).
 void time_passes(System& s, bool c64, int reps, hipStream_t st, double* ms5);
 // same over a batch of systems in one launch per pass
 void time_passes_batch(const std::vector<System*>& v, bool c64, int reps, hipStream_t st, double* ms5);

@@ -14,6 +14,12 @@ namespace tdda {
 
 namespace {
 
+// A tuned choice leaves the default (variant 0, no fused y/z pass, plane
+// passes on) only when the alternative is faster by this factor: timings of
+// a few repetitions differ by a few per cent from run to run, and near-ties
+// would otherwise flip between runs.
+constexpr double kTuneMargin = 0.97;
+
 const char* const kPrec[2] = {"c64", "c128"};
 const char* const kPass[NPASS] = {"xinv", "y", "z", "xfwd"};
 
@@ -62,6 +68,18 @@ bool Schedule::fused(int ly, int lz) {
     return it != fused_.end() && it->second != 0;
 }
 
+bool Schedule::plane(int L) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = plane_.find(L);
+    return it == plane_.end() || it->second != 0;
+}
+
+void Schedule::set_plane(int L, bool on) {
+    TDDA_REQUIRE(L >= 1, ERR_INVALID, "schedule length must be positive");
+    std::lock_guard<std::mutex> lk(mu_);
+    plane_[L] = on ? 1 : 0;
+}
+
 void Schedule::set_variant(int pr, int pass, int L, int v) {
     check_variant(pr, pass, L, v);
     std::lock_guard<std::mutex> lk(mu_);
@@ -78,6 +96,7 @@ void Schedule::clear() {
     std::lock_guard<std::mutex> lk(mu_);
     var_.clear();
     fused_.clear();
+    plane_.clear();
 }
 
 std::string Schedule::text() {
@@ -88,12 +107,14 @@ std::string Schedule::text() {
         o << "var " << kPrec[std::get<0>(kv.first)] << " " << kPass[std::get<1>(kv.first)] << " "
           << std::get<2>(kv.first) << " " << kv.second << "\n";
     for (const auto& kv : fused_) o << "yz " << kv.first.first << " " << kv.first.second << " " << kv.second << "\n";
+    for (const auto& kv : plane_) o << "xy " << kv.first << " " << kv.second << "\n";
     return o.str();
 }
 
 void Schedule::parse(const std::string& text) {
     std::map<std::tuple<int, int, int>, int> var;
     std::map<std::pair<int, int>, int> fused;
+    std::map<int, int> plane;
     std::istringstream in(text);
     std::string line;
     int lineno = 0;
@@ -124,6 +145,11 @@ void Schedule::parse(const std::string& text) {
             TDDA_REQUIRE((ls >> ly >> lz >> on) && !(ls >> extra) && ly >= 1 && lz >= 1 && (on == 0 || on == 1),
                          ERR_INVALID, where + "expected 'yz <Ly> <Lz> <0|1>'");
             fused[std::make_pair(ly, lz)] = on;
+        } else if (kind == "xy") {
+            int L = 0, on = -1;
+            TDDA_REQUIRE((ls >> L >> on) && !(ls >> extra) && L >= 1 && (on == 0 || on == 1), ERR_INVALID,
+                         where + "expected 'xy <L> <0|1>'");
+            plane[L] = on;
         } else {
             throw Error(ERR_INVALID, where + "unknown entry '" + kind + "'");
         }
@@ -131,6 +157,7 @@ void Schedule::parse(const std::string& text) {
     std::lock_guard<std::mutex> lk(mu_);
     var_.swap(var);
     fused_.swap(fused);
+    plane_.swap(plane);
 }
 
 void Schedule::load(const std::string& path) {
@@ -151,6 +178,7 @@ void Schedule::save(const std::string& path) {
 
 int schedule_variant(int pr, int pass, int L) { return Schedule::instance().variant(pr, pass, L); }
 bool schedule_fused(int ly, int lz) { return Schedule::instance().fused(ly, lz); }
+bool schedule_plane(int L) { return Schedule::instance().plane(L); }
 
 // ---------------------------------------------------------------------------
 // tuner
@@ -162,8 +190,12 @@ TuneReport tune(const std::vector<std::array<int, 3>>& boxes, int count, int rep
     Schedule& S = Schedule::instance();
     struct RestoreMode {
         int mode;
-        ~RestoreMode() { set_fused_yz(mode); }
-    } restore{fused_yz()};
+        bool planes;
+        ~RestoreMode() {
+            set_fused_yz(mode);
+            set_plane_passes(planes);
+        }
+    } restore{fused_yz(), plane_passes()};
     std::mt19937_64 rng(20241204);
     std::uniform_real_distribution<double> U(0.0, 1.0);
     const SiciDevice none{};   // LDR: no tables
@@ -208,9 +240,10 @@ TuneReport tune(const std::vector<std::array<int, 3>>& boxes, int count, int rep
     };
 
     TuneReport rep;
-    // 1. split variants of every pass, with the fused pass off (c64 Y and Z
-    //    timings are those of the three-pass path)
+    // 1. split variants of every pass, with the fused and plane passes off
+    //    (c64 X, Y and Z timings are those of the separate passes)
     set_fused_yz(FUSED_OFF);
+    set_plane_passes(false);
     for (int pr = 0; pr < 2; ++pr) {
         std::map<std::pair<int, int>, std::array<double, 2>> acc;   // (pass, L) -> ms per variant
         for (int v = 0; v < 2; ++v) {
@@ -243,7 +276,7 @@ TuneReport tune(const std::vector<std::array<int, 3>>& boxes, int count, int rep
         }
         for (const auto& kv : acc) {
             const int p = kv.first.first, L = kv.first.second;
-            const int chosen = (kv.second[1] >= 0 && kv.second[1] < kv.second[0]) ? 1 : 0;
+            const int chosen = (kv.second[1] >= 0 && kv.second[1] < kTuneMargin * kv.second[0]) ? 1 : 0;
             if (split_variants(p, L) > 1) S.set_variant(pr, p, L, chosen);
             rep.passes.push_back(TunePass{pr, p, L, {kv.second[0], kv.second[1]}, chosen});
         }
@@ -267,9 +300,31 @@ TuneReport tune(const std::vector<std::array<int, 3>>& boxes, int count, int rep
         }
     }
     for (const auto& kv : facc) {
-        const bool on = kv.second[1] < kv.second[0];
+        const bool on = kv.second[1] < kTuneMargin * kv.second[0];
         S.set_fused(kv.first.first, kv.first.second, on);
         rep.fused.push_back(TuneFused{kv.first.first, kv.first.second, kv.second[0], kv.second[1], on});
+    }
+    // 3. the plane passes per plane length against the choices above (the
+    //    y/z choice as recorded, plane passes off), whole c64 matvec
+    set_fused_yz(FUSED_SCHEDULE);
+    std::map<int, std::array<double, 2>> pacc;
+    for (const auto& b : boxes) {
+        int L[NPASS];
+        lengths(b, L);
+        if (plane_class_of(L[PASS_X], L[PASS_Y], b[0], b[1]) < 0) continue;
+        auto& a = pacc[L[PASS_X]];
+        for (int on = 0; on < 2; ++on) {
+            set_plane_passes(on != 0);
+            S.set_plane(L[PASS_X], on != 0);
+            double ms5[5];
+            time5(b, true, ms5);
+            a[on] += ms5[0] + ms5[1] + ms5[2] + ms5[3] + ms5[4];
+        }
+    }
+    for (const auto& kv : pacc) {
+        const bool on = !(kv.second[0] < kTuneMargin * kv.second[1]);
+        S.set_plane(kv.first, on);
+        rep.planes.push_back(TunePlane{kv.first, kv.second[0], kv.second[1], on});
     }
     return rep;
 }
@@ -279,16 +334,20 @@ std::string tune_report_text(const TuneReport& r) {
     char buf[160];
     for (const auto& p : r.passes) {
         if (p.ms[1] >= 0)
-            std::snprintf(buf, sizeof buf, "pass %s %s %d ms %.5f %.5f chosen %d\n", kPrec[p.pr], kPass[p.pass], p.L,
+            std::snprintf(buf, sizeof buf, "pass %s %s %d ms %.7f %.7f chosen %d\n", kPrec[p.pr], kPass[p.pass], p.L,
                           p.ms[0], p.ms[1], p.chosen);
         else
-            std::snprintf(buf, sizeof buf, "pass %s %s %d ms %.5f - chosen %d\n", kPrec[p.pr], kPass[p.pass], p.L,
+            std::snprintf(buf, sizeof buf, "pass %s %s %d ms %.7f - chosen %d\n", kPrec[p.pr], kPass[p.pass], p.L,
                           p.ms[0], p.chosen);
         o << buf;
     }
     for (const auto& f : r.fused) {
-        std::snprintf(buf, sizeof buf, "yz %d %d ms %.5f %.5f chosen %d\n", f.ly, f.lz, f.ms_three, f.ms_fused,
+        std::snprintf(buf, sizeof buf, "yz %d %d ms %.7f %.7f chosen %d\n", f.ly, f.lz, f.ms_three, f.ms_fused,
                       f.chosen ? 1 : 0);
+        o << buf;
+    }
+    for (const auto& p : r.planes) {
+        std::snprintf(buf, sizeof buf, "xy %d ms %.7f %.7f chosen %d\n", p.L, p.ms_off, p.ms_on, p.chosen ? 1 : 0);
         o << buf;
     }
     return o.str();

@@ -329,6 +329,8 @@ def tune(boxes, count=4, reps=10):
         if f[0] == "pass":
             out.append({"kind": "pass", "precision": f[1], "pass": f[2], "L": int(f[3]), "ms": float(f[5]),
                         "ms_alt": None if f[6] == "-" else float(f[6]), "chosen": int(f[8])})
+        elif f[0] == "xy":
+            out.append({"kind": "xy", "L": int(f[1]), "ms_off": float(f[3]), "ms_on": float(f[4]), "chosen": int(f[6])})
         else:
             out.append({"kind": "yz", "Ly": int(f[1]), "Lz": int(f[2]), "ms_three": float(f[4]),
                         "ms_fused": float(f[5]), "chosen": int(f[7])})
