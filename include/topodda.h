@@ -307,6 +307,9 @@ typedef struct {
     /* host time of the loop thread (ms): admissions, waits for a chunk,
      * retirements, batch builds (maps and joins) */
     double host_admit_ms, host_wait_ms, host_retire_ms, host_build_ms;
+    /* loop thread: enqueueing chunks (ms), and chunks replayed from a graph */
+    double host_enqueue_ms;
+    long graph_chunks;
 } tdda_server_stats;
 
 int tdda_server_create(const tdda_server_opts* o, const double* si, const double* ci, long nsici, double delta,

@@ -647,6 +647,8 @@ int tdda_server_get_stats(tdda_server* s, tdda_server_stats* out) {
         out->host_wait_ms = st.host_ms[1];
         out->host_retire_ms = st.host_ms[2];
         out->host_build_ms = st.host_ms[3];
+        out->host_enqueue_ms = st.enqueue_ms;
+        out->graph_chunks = st.graph_chunks;
     });
 }
 

@@ -374,7 +374,7 @@ void Server::drop_graph(int b) {
 }
 
 // A chunk of iterations of batch b and the end-of-chunk update, on main_.
-void Server::enqueue_chunk(int b) {
+bool Server::enqueue_chunk(int b) {
     const TwiddlePools tw = PlanRegistry::instance().pools();
     const LaunchSet& ls = batch_[b].ls;
     auto body = [&](hipStream_t st) {
@@ -385,9 +385,10 @@ void Server::enqueue_chunk(int b) {
     if (graph_capture() && batch_uses_[b] >= 4) {   // recording pays back over a few replays
         if (!chunk_graph_[b]) chunk_graph_[b] = capture_graph(body);
         TDDA_HIP(hipGraphLaunch(chunk_graph_[b], main_));
-    } else {
-        body(main_);
+        return true;
     }
+    body(main_);
+    return false;
 }
 
 void Server::loop() {
@@ -396,12 +397,15 @@ void Server::loop() {
         while (true) {
             // 1. enqueue a chunk of iterations for batch `cur`
             if (!members_[cur].empty()) {
-                enqueue_chunk(cur);
+                const double te = now();
+                const bool replay = enqueue_chunk(cur);
                 TDDA_HIP(hipMemcpyAsync(snap_[cur], ks_.get(), opt_.max_slots * sizeof(KScalars),
                                         hipMemcpyDeviceToHost, main_));
                 TDDA_HIP(hipEventRecord(chunk_done_[cur], main_));
                 inflight_[cur] = true;
                 std::lock_guard<std::mutex> lk(mu_);
+                stats_.enqueue_ms += 1e3 * (now() - te);
+                if (replay) stats_.graph_chunks++;
                 stats_.chunks++;
                 stats_.launches += (long)opt_.chunk * (opt_.refined ? 18 : 12) + 3;
             }

@@ -61,6 +61,8 @@ struct ServerStats {
     // host ms: admission thread (admit), loop thread (wait for a chunk,
     // retire, join and rebuild)
     double host_ms[4] = {0, 0, 0, 0};
+    double enqueue_ms = 0;     // loop thread: enqueueing chunks (launches or graph replays)
+    long graph_chunks = 0;     // chunks replayed from a captured graph
 };
 
 class Server {
@@ -141,7 +143,7 @@ class Server {
     long batch_version_[2] = {-1, -1};
     int batch_uses_[2] = {0, 0};
     hipGraphExec_t chunk_graph_[2] = {nullptr, nullptr};
-    void enqueue_chunk(int b);
+    bool enqueue_chunk(int b);   // true: replayed from the batch's graph
     void drop_graph(int b);
     Batch batch_[2], init_batch_[2];
     // Admitted requests whose operators and densities are still being set up

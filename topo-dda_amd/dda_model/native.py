@@ -50,7 +50,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_long), ("completed", ctypes.c_long), ("chunks", ctypes.c_long),
                 ("rebuilds", ctypes.c_long), ("launches", ctypes.c_long), ("max_active", ctypes.c_int),
                 ("peak_bytes", ctypes.c_long), ("host_admit_ms", ctypes.c_double), ("host_wait_ms", ctypes.c_double),
-                ("host_retire_ms", ctypes.c_double), ("host_build_ms", ctypes.c_double)]
+                ("host_retire_ms", ctypes.c_double), ("host_build_ms", ctypes.c_double),
+                ("host_enqueue_ms", ctypes.c_double), ("graph_chunks", ctypes.c_long)]
 
 
 def lib():
