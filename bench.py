@@ -79,14 +79,14 @@ def percentile(v, q):
 
 
 def run_stream(native, reqs, precision, max_slots, chunk, arrivals=None, fields=False, timeout=600.0, tol=1e-5,
-               err_tol=0.0, precond="none", server=None, id0=0):
+               err_tol=0.0, precond="none", server=None, id0=0, ru_delta=0.01):
     """Submit every request (at its arrival time if given) and wait for all.
     A given server is used and kept open (request ids id0, id0 + 1, ...);
     otherwise one is made for the stream and closed.  Returns (wall seconds
     from first submission to last completion, results by request index,
     server statistics, fields or None)."""
     srv = server or native.Server(precision=precision, max_slots=max_slots, chunk=chunk, keep_fields=fields,
-                                  tol=tol, err_tol=err_tol, precond=precond)
+                                  tol=tol, err_tol=err_tol, precond=precond, ru_delta=ru_delta)
     results = {}
     t0 = time.perf_counter()
     nxt = 0
@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=2)
     ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--ru-delta", type=float, default=0.01,
+                    help="reliable update when the recursive residual falls this far below its peak since the last")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the 128x128x64 multi-RHS record")
     ap.add_argument("--precond", default="jacobi", choices=["jacobi", "none"],
@@ -414,7 +416,7 @@ def main():
     # service would; every step still submits R fresh requests and waits for
     # all of them (ramp-up and tail included).
     server = native.Server(precision="fp32_refined", max_slots=a.slots, chunk=a.chunk, keep_fields=False,
-                           precond=a.precond)
+                           precond=a.precond, ru_delta=a.ru_delta)
     next_id = 0
     # warmup: requests outside the timed seeds (plans, twiddles, allocator,
     # system pool, code objects)
@@ -465,7 +467,8 @@ def main():
     # step 1: reference fields from a tight FP64 solve (batched, tol 1e-10)
     sub = reqs0[:min(a.fp64_requests, R)]
     _, res_t, _, f_tight = run_stream(native, sub, "fp64", len(sub), 16, fields=True, tol=1e-10, precond="jacobi")
-    _, res32s, _, f32 = run_stream(native, sub, "fp32_refined", a.slots, a.chunk, fields=True, precond=a.precond)
+    _, res32s, _, f32 = run_stream(native, sub, "fp32_refined", a.slots, a.chunk, fields=True, precond=a.precond,
+                                   ru_delta=a.ru_delta)
     err32 = [rel_err(f32[i], f_tight[i]) for i in f_tight]
     est = [res32s[i]["err_est"] for i in f_tight]
     # the FP64 single-solve path (one request at a time, FP64 transforms and

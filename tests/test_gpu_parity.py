@@ -407,6 +407,15 @@ def test_fused_yz_apply_matches_three_passes(dm, O, tables):
     three-pass c64 path and the FP64 apply on the same operators, in one
     mixed batch; then refined solves with and without it."""
     from dda_model import native
+    # the plane passes take precedence over the fused y/z pass: off here
+    native.set_plane_passes(False)
+    try:
+        _fused_yz_body(native, O)
+    finally:
+        native.set_plane_passes(True)
+
+
+def _fused_yz_body(native, O):
     _log("[gpu] fused y/z apply " + str(FUSED_SHAPES))
     rng = np.random.default_rng(21)
     probs, reqs = [], []

@@ -155,6 +155,7 @@ Model::Model(const ModelConfig& cfg) : cfg_(cfg) {
     }
     P_.assign(3 * (size_t)N_, cd(0, 0));
     eresult_.assign(3 * (size_t)N_, cd(0, 0));
+    upload_obj_maps();
     update_alpha();
     if (cfg.verbose) {
         std::printf("DDAModel: %d dipoles, %d free parameters, FFT grid %dx%dx%d, K=%.9g, operator %s (%.1f ms)\n",
@@ -172,14 +173,61 @@ cd Model::eps_of(double x) const {
     return material_[lf] + (x - (double)lf) * (material_[ln] - material_[lf]);
 }
 
+// The objective's static maps on the device: region mask, dipole ->
+// parameter and parameter -> dipoles (in p2g_ order), and zeroed buffers for
+// the solution and the partials.
+void Model::upload_obj_maps() {
+    hipStream_t st = default_stream();
+    const size_t n3 = 3 * (size_t)N_;
+    std::vector<unsigned char> reg(region_.begin(), region_.end());
+    std::vector<int> off(nfpara_ + 1, 0), dips;
+    dips.reserve(N_);
+    for (int i = 0; i < nfpara_; ++i) {
+        dips.insert(dips.end(), p2g_[i].begin(), p2g_[i].end());
+        off[i + 1] = (int)dips.size();
+    }
+    d_region_.alloc(N_);
+    d_gpara_.alloc(N_);
+    d_off_.alloc(nfpara_ + 1);
+    d_dips_.alloc(dips.size());
+    d_region_.upload(reg.data(), reg.size(), st);
+    d_gpara_.upload(gpara_.data(), N_, st);
+    d_off_.upload(off.data(), off.size(), st);
+    d_dips_.upload(dips.data(), dips.size(), st);
+    d_dal_.alloc(N_);
+    d_P_.alloc(n3);
+    d_P_.zero(st);
+    d_w_.alloc(n3);
+    d_part_.alloc((N_ + 255) / 256);
+    d_E_.alloc(1);
+    d_devx_.alloc(nfpara_);
+    d_grad_.alloc(nfpara_);
+    d_step_.alloc(nfpara_);
+    d_devp_.alloc(n3);
+    d_adevxp_.alloc(n3);
+    d_alt_.alloc(nfpara_);
+    TDDA_HIP(hipStreamSynchronize(st));   // the host vectors leave scope
+    obj_maps_.region = d_region_.get();
+    obj_maps_.gpara = d_gpara_.get();
+    obj_maps_.off = d_off_.get();
+    obj_maps_.dips = d_dips_.get();
+}
+
 // DDAModel.cpp:1122-1135
 void Model::update_alpha() {
     diel_.assign(N_, cd(0, 0));
     al_.assign(N_, cd(0, 0));
+    std::vector<cd> dal(N_);
     for (int i = 0; i < N_; ++i) {
         diel_[i] = eps_of(diel_old_[3 * i]);
         al_[i] = ldr_inverse_alpha(K_, cfg_.d, diel_[i], cfg_.ne0, cfg_.nk);
+        // d(1/alpha)/d rho of the exact gradient: eps is linear in rho up to 1
+        const double x = diel_old_[3 * i];
+        const cd slope = x <= 1.0 && material_.size() >= 2 ? material_[1] - material_[0] : cd(0, 0);
+        dal[i] = ldr_inverse_alpha_deps(K_, cfg_.d, diel_[i], cfg_.ne0, cfg_.nk) * slope;
     }
+    d_dal_.upload(reinterpret_cast<const double2*>(dal.data()), N_, default_stream());
+    TDDA_HIP(hipStreamSynchronize(default_stream()));
     if (sys64_) sys64_->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
     if (sys32_) {
         sys32_->set_alpha(reinterpret_cast<const double2*>(al_.data()), default_stream());
@@ -230,11 +278,17 @@ System& Model::system() {
     return *s;
 }
 
-SolveResult Model::run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit, double tol, std::vector<cd>& P) {
+SolveResult Model::run_solve(const std::vector<cd>* rhs, const double2* rhs_dev, const cd* P0, int maxit, double tol,
+                             std::vector<cd>& P) {
     System& s = system();
     hipStream_t st = default_stream();
-    s.set_rhs(reinterpret_cast<const double2*>(rhs.data()), st);
-    rhs_on_device_[precision_] = &rhs == &einc_;
+    if (rhs) {
+        s.set_rhs(reinterpret_cast<const double2*>(rhs->data()), st);
+        rhs_on_device_[precision_] = rhs == &einc_;
+    } else {
+        TDDA_HIP(hipMemcpyAsync(s.rhs.get(), rhs_dev, 3 * (size_t)N_ * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        rhs_on_device_[precision_] = false;
+    }
     bool warm = false;
     if (P0) {
         for (long i = 0; i < 3L * N_; ++i)
@@ -252,6 +306,7 @@ SolveResult Model::run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit
     o.warm = warm;
     std::vector<System*> v{&s};
     SolveResult r = solve_batch(v, o, st)[0];
+    TDDA_HIP(hipMemcpyAsync(d_P_.get(), s.P.get(), 3 * (size_t)N_ * sizeof(double2), hipMemcpyDeviceToDevice, st));
     P.assign(3 * (size_t)N_, cd(0, 0));
     s.get_solution(reinterpret_cast<double2*>(P.data()), st);
     TDDA_HIP(hipStreamSynchronize(st));
@@ -265,111 +320,60 @@ SolveResult Model::run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit
 // DDAModel.cpp:874-885 + 1153-1192
 void Model::solve_electric_field(const cd* P0, long n, int maxit, double tol) {
     TDDA_REQUIRE(n == 3L * N_, ERR_INVALID, "originPolarization must hold 3*num_total values");
-    last_ = run_solve(einc_, P0, maxit, tol, P_);
+    last_ = run_solve(&einc_, nullptr, P0, maxit, tol, P_);
     for (int i = 0; i < N_; ++i) {
         const cd lf = (2.0 + diel_[i]) / 3.0;
         for (int c = 0; c < 3; ++c) eresult_[3 * i + c] = al_[i] * P_[3 * i + c] / lf;
     }
 }
 
-// ObjDDAModel.cpp:205-212 (GetVal = Reset + N SingleResponse + log)
-double Model::calculate_objective() const {
-    double eint = 0.0;
-    for (int i = 0; i < N_; ++i) {
-        if (!region_[i]) continue;
-        for (int c = 0; c < 3; ++c) eint += std::pow(std::abs(al_[i] * P_[3 * i + c]), pownum_);
-    }
-    return std::log(eint);
+// w = |al P|^p and E = sum of the region's w for the current solution, on the
+// device (ObjDDAModel.cpp:130-212)
+void Model::objective_terms(hipStream_t st) {
+    launch_obj_terms(N_, pownum_, system().al64.get(), d_P_.get(), d_region_.get(), d_w_.get(), d_part_.get(),
+                     d_E_.get(), st);
 }
 
-// DDAModel.cpp:398-578
+// ObjDDAModel.cpp:205-212 (GetVal = Reset + N SingleResponse + log)
+double Model::calculate_objective() {
+    hipStream_t st = default_stream();
+    objective_terms(st);
+    double E = 0.0;
+    d_E_.download(&E, 1, st);
+    TDDA_HIP(hipStreamSynchronize(st));
+    return std::log(E);
+}
+
+// DDAModel.cpp:398-578: the explicit partials and the assembly run on the
+// device (model_obj.hip); the host forms only the perturbed 1/alpha of each
+// parameter for the reference's one-sided differences (the material table
+// interpolation of update_alpha).
 std::vector<double> Model::calculate_gradients(double eps, double origin, int maxit, double tol) {
-    const int p = pownum_;
-    const size_t n3 = 3 * (size_t)N_;
-    std::vector<double> w(n3, 0.0);
-    double eint = 0.0;
-    for (int i = 0; i < N_; ++i) {
-        for (int c = 0; c < 3; ++c) {
-            w[3 * i + c] = std::pow(std::abs(al_[i] * P_[3 * i + c]), p);
-            if (region_[i]) eint += w[3 * i + c];
-        }
-    }
-    std::vector<double> devx(nfpara_, 0.0);
-    std::vector<cd> adevxp(n3, cd(0, 0)), devp(n3, cd(0, 0));
-    if (grad_mode_ == GRAD_REFERENCE) {
-        // One-sided FD in rho with the reference's sign rule (DDAModel.cpp:482-491) and FD
-        // in Re P / Im P (:538-578); each quotient is formed from one perturbed sum,
-        // log(E + D) - origin = (log E - origin) + log1p(D/E).
-        const double base = std::log(eint) - origin;
+    hipStream_t st = default_stream();
+    objective_terms(st);
+    ObjElementParams A{N_, nfpara_, pownum_, grad_mode_ == GRAD_EXACT ? 1 : 0, eps, origin};
+    if (!A.exact) {
+        // one-sided difference with the reference's sign rule (DDAModel.cpp:482-491)
+        std::vector<cd> alt(nfpara_);
+        std::vector<double> step(nfpara_);
         for (int i = 0; i < nfpara_; ++i) {
             const double x0 = params_[i];
-            const int sign = x0 >= eps ? -1 : 1;
-            const double step = sign * eps;
-            const cd alt = ldr_inverse_alpha(K_, cfg_.d, eps_of(x0 + step), cfg_.ne0, cfg_.nk);
-            double delta = 0.0;
-            for (int dip : p2g_[i]) {
-                if (!region_[dip]) continue;
-                for (int c = 0; c < 3; ++c)
-                    delta += std::pow(std::abs(alt * P_[3 * dip + c]), p) - w[3 * dip + c];
-            }
-            devx[i] = (base + std::log1p(delta / eint)) / step;
-            for (int dip : p2g_[i]) {
-                const cd change = (alt - al_[dip]) / step;
-                for (int c = 0; c < 3; ++c) adevxp[3 * dip + c] = change * P_[3 * dip + c];
-            }
+            step[i] = (x0 >= eps ? -1 : 1) * eps;
+            alt[i] = ldr_inverse_alpha(K_, cfg_.d, eps_of(x0 + step[i]), cfg_.ne0, cfg_.nk);
         }
-        for (int i = 0; i < N_; ++i) {
-            for (int c = 0; c < 3; ++c) {
-                const size_t k = 3 * i + c;
-                double dr = 0.0, di = 0.0;
-                if (region_[i]) {
-                    dr = std::pow(std::abs(al_[i] * (P_[k] + eps)), p) - w[k];
-                    di = std::pow(std::abs(al_[i] * (P_[k] + cd(0.0, eps))), p) - w[k];
-                }
-                const double ar = (base + std::log1p(dr / eint)) / eps;
-                const double ai = (base + std::log1p(di / eint)) / eps;
-                devp[k] = cd(ar, -ai);   // d/dReP + d/dImP / i
-            }
-        }
-    } else {
-        // exact partials: f = log E, E = sum |al P|^p
-        for (int i = 0; i < nfpara_; ++i) {
-            for (int dip : p2g_[i]) {
-                const double x = diel_old_[3 * dip];
-                const cd slope = x <= 1.0 && material_.size() >= 2 ? material_[1] - material_[0] : cd(0, 0);
-                const cd dal = ldr_inverse_alpha_deps(K_, cfg_.d, diel_[dip], cfg_.ne0, cfg_.nk) * slope;
-                for (int c = 0; c < 3; ++c) {
-                    const cd Pk = P_[3 * dip + c];
-                    adevxp[3 * dip + c] = dal * Pk;
-                    if (!region_[dip]) continue;
-                    const cd wk = al_[dip] * Pk;
-                    const double aw = std::abs(wk);
-                    if (aw == 0.0) continue;
-                    devx[i] += p * std::pow(aw, p - 2) * std::real(std::conj(wk) * Pk * dal) / eint;
-                }
-            }
-        }
-        for (int i = 0; i < N_; ++i) {
-            if (!region_[i]) continue;
-            for (int c = 0; c < 3; ++c) {
-                const cd wk = al_[i] * P_[3 * i + c];
-                const double aw = std::abs(wk);
-                if (aw == 0.0 && p < 2) continue;
-                devp[3 * i + c] = p * std::pow(aw, p - 2) * std::conj(wk) * al_[i] / eint;
-            }
-        }
+        d_alt_.upload(reinterpret_cast<const double2*>(alt.data()), nfpara_, st);
+        d_step_.upload(step.data(), nfpara_, st);
+        TDDA_HIP(hipStreamSynchronize(st));   // the host vectors leave scope
     }
-    // adjoint solve (A is complex symmetric): lambda = A^-1 devp, from zero
-    std::vector<cd> lambda;
-    last_adj_ = run_solve(devp, nullptr, maxit, tol, lambda);
+    launch_obj_partials(A, obj_maps_, system().al64.get(), d_P_.get(), d_w_.get(), d_E_.get(), d_dal_.get(),
+                        d_alt_.get(), d_step_.get(), d_devp_.get(), d_adevxp_.get(), d_devx_.get(), st);
+    // adjoint solve (A is complex symmetric): lambda = A^-1 devp, from zero;
+    // the reference leaves lambda in P (DDAModel.cpp:433)
+    last_adj_ = run_solve(nullptr, d_devp_.get(), nullptr, maxit, tol, P_);
+    launch_obj_assemble(nfpara_, obj_maps_, d_devx_.get(), d_P_.get(), d_adevxp_.get(), d_grad_.get(), st);
     std::vector<double> grad(nfpara_, 0.0);
-    for (int i = 0; i < nfpara_; ++i) {
-        cd m(0, 0);
-        for (int dip : p2g_[i])
-            for (int c = 0; c < 3; ++c) m += lambda[3 * dip + c] * adevxp[3 * dip + c];
-        grad[i] = devx[i] - m.real();
-    }
-    P_ = lambda;   // the reference leaves the adjoint solution in P (DDAModel.cpp:433)
+    d_grad_.download(grad.data(), nfpara_, st);
+    TDDA_HIP(hipStreamSynchronize(st));
     return grad;
 }
 

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "model_obj.h"
 #include "runtime.h"
 
 namespace tdda {
@@ -52,7 +53,7 @@ class Model {
 
     // bound API (python_bindings/PythonBindings.cpp:61-69)
     void solve_electric_field(const cd* P0, long n, int maxit, double tol);
-    double calculate_objective() const;
+    double calculate_objective();
     std::vector<double> calculate_gradients(double eps, double origin, int maxit, double tol);
     void set_parameters(const double* v, long n);
     const std::vector<double>& parameters() const { return params_; }
@@ -84,7 +85,14 @@ class Model {
     void update_alpha();
     void upload_field_factor(System& s);
     cd eps_of(double x) const;
-    SolveResult run_solve(const std::vector<cd>& rhs, const cd* P0, int maxit, double tol, std::vector<cd>& P);
+    // solve from the host right-hand side rhs, or from rhs_dev on the device
+    // when rhs is null; P receives the solution, and so does d_P_
+    SolveResult run_solve(const std::vector<cd>* rhs, const double2* rhs_dev, const cd* P0, int maxit, double tol,
+                          std::vector<cd>& P);
+    // w, E of the current solution on the device (objective_terms) and the
+    // model's static maps there (upload_obj_maps)
+    void objective_terms(hipStream_t st);
+    void upload_obj_maps();
     System& system();
 
     ModelConfig cfg_;
@@ -112,6 +120,14 @@ class Model {
     double op_setup_ms_ = 0.0;
     bool op_hit_ = false;
     bool rhs_on_device_[2] = {false, false};
+    // device side of the objective and the gradient partials
+    ObjDeviceMaps obj_maps_;
+    DevBuf<unsigned char> d_region_;
+    DevBuf<int> d_gpara_, d_off_, d_dips_;
+    DevBuf<double2> d_dal_;       // exact mode: d(1/alpha)/d rho per dipole
+    DevBuf<double2> d_P_;         // the last solution (forward or adjoint), 3N
+    DevBuf<double> d_w_, d_part_, d_E_, d_devx_, d_grad_, d_step_;
+    DevBuf<double2> d_devp_, d_adevxp_, d_alt_;
 };
 
 }  // namespace tdda
