@@ -296,6 +296,26 @@ def smooth_density(shape, sigma, seed):
     return np.clip(0.5 + 0.25 * f / f.std(), 0.0, 1.0).reshape(-1)
 
 
+def multi_rhs_probe(native, reps):
+    """The 128x128x64 eight-RHS batch's c64 matvec (ms per pass), with its
+    right-hand sides sharing 4 operators (4 wavelengths x 2 polarisations,
+    R_op = 2: the Z tiles of systems that share an operator are interleaved so
+    that each octant record is read from HBM about once per operator) and,
+    for comparison, with 8 distinct operators (8 wavelengths)."""
+    rho = smooth_density(C4_SHAPE, 3.0, seed=5)
+    shared = [native.Request(shape=C4_SHAPE, d=8.0, lam=l, rho=rho, n_k=(0.0, 0.0, 1.0), n_e0=e)
+              for l in C4_LAMS for e in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))]
+    distinct = [native.Request(shape=C4_SHAPE, d=8.0, lam=450.0 + 40.0 * k, rho=rho, n_k=(0.0, 0.0, 1.0),
+                               n_e0=(1.0, 0.0, 0.0)) for k in range(8)]
+    out = {}
+    for name, reqs in (("shared", shared), ("distinct", distinct)):
+        probs = [native.Problem(r, precision="fp32_refined") for r in reqs]
+        out[name] = [float(x) for x in native.time_passes(probs, c64=True, reps=reps)]
+        del probs
+    native.check(native.lib().tdda_trim_caches())
+    return out
+
+
 def multi_rhs_record(native, reps):
     """128x128x64 dipoles (FFT 256x256x128), d = 8 nm, smooth density (sigma
     3), 8 right-hand sides = 4 wavelengths x 2 polarisations, 2 per operator.
@@ -329,6 +349,9 @@ def multi_rhs_record(native, reps):
                                  "algorithmic_bytes": int(b_mv), "GBs": round(b_mv / t / 1e9, 1),
                                  "frac": round(b_mv / t / 1e9 / HBM_PEAK_GBS, 4)}
             del probs
+    mp = multi_rhs_probe(native, reps)
+    out["matvec"]["z_ms_shared_operators"] = round(mp["shared"][2], 4)
+    out["matvec"]["z_ms_distinct_operators"] = round(mp["distinct"][2], 4)
     out["speedup_refined_vs_fp64_batched_jacobi"] = round(
         out["fp64_jacobi"]["seconds"] / out["fp32_refined_jacobi"]["seconds"], 3)
     native.check(native.lib().tdda_trim_caches())
@@ -375,6 +398,8 @@ def main():
                     help="only the batched pass timing and a 256 MiB copy (for counter runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default kernel schedule (no autotuning in the warm-up)")
+    ap.add_argument("--multi-rhs-only", action="store_true",
+                    help="only the 128x128x64 eight-RHS matvec, shared and distinct operators (for counter runs)")
     ap.add_argument("--saturation-only", action="store_true",
                     help="only warm-up and the timed saturation steps; prints solves/s (for kernel traces)")
     a = ap.parse_args()
@@ -383,6 +408,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from dda_model import native
+
+    if a.multi_rhs_only:
+        if not a.no_tune:
+            native.tune(SHAPES, count=16, reps=10)
+        print(json.dumps({"multi_rhs_pass_ms": multi_rhs_probe(native, 30)}), flush=True)
+        return
 
     if a.roofline_only:
         # the matvec probe alone (after the tuner, as in the full run), so that
