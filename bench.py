@@ -168,6 +168,8 @@ def cpu_baseline(reqs, iterations, workers=1, per_shape_s=4.0):
 
 
 def lattice_sum_rates(native, tables, threads):
+    # tables: the package's Si/Ci tables for the device set-up; the CPU
+    # oracle evaluates its own
     """FP64 Green's-tensor lattice sum of the periodic unit cell (32x32x12,
     d = 10 nm, 600 nm, FCD, period 32 cells), in Green evaluations per second
     = (2Nx-1)(2Ny-1)(2Nz-1)(2 MAXm+1)(2 MAXn+1) / time.  CPU: the oracle's C
@@ -180,8 +182,9 @@ def lattice_sum_rates(native, tables, threads):
     si, ci, delta = tables
     K = 2 * math.pi / lam
     cpu_images = 8
+    osi, oci, odelta = O.sici_tables()
     t0 = time.perf_counter()
-    O.green_offsets(*shape, d, K, O.METHOD_FCD, cpu_images, cpu_images, 32 * d, 32 * d, si, ci, delta,
+    O.green_offsets(*shape, d, K, O.METHOD_FCD, cpu_images, cpu_images, 32 * d, 32 * d, osi, oci, odelta,
                     octant_only=True)
     t_cpu = time.perf_counter() - t0
     cpu_rate = m_ref * (2 * cpu_images + 1) ** 2 / t_cpu
@@ -354,6 +357,72 @@ def multi_rhs_record(native, reps):
     out["matvec"]["z_ms_distinct_operators"] = round(mp["distinct"][2], 4)
     out["speedup_refined_vs_fp64_batched_jacobi"] = round(
         out["fp64_jacobi"]["seconds"] / out["fp32_refined_jacobi"]["seconds"], 3)
+    native.check(native.lib().tdda_trim_caches())
+    return out
+
+
+def periodic_cell_record(native, tables):
+    """BASELINE c1: 32x32x12 unit cell, d = 10 nm, periods 320 nm, FCD with
+    +-50 images, 600 nm, normal incidence, y-polarised, smooth density
+    (Gaussian-filtered noise, sigma 2, clipped).  Reports the FP64 operator
+    set-up (lattice sum, DFTs), the refined (c64 + FP64 reliable updates) and
+    FP64 solves to 1e-5 (solver ms, iterations) and their fields against a
+    1e-10 FP64 solve."""
+    shape = (32, 32, 12)
+    rho = smooth_density(shape, 2.0, seed=1)
+    req = native.Request(shape=shape, d=10.0, lam=600.0, rho=rho, n_k=(0.0, 0.0, 1.0), n_e0=(0.0, 1.0, 0.0),
+                         method="FCD", images=50, period=(32, 32))
+    native.check(native.lib().tdda_trim_caches())
+    p64 = native.Problem(req, precision="fp64", tables=tables)
+    setup_ms = p64.operator_setup_ms()
+    p64.solve(tol=1e-10)
+    e_ref = p64.field()
+    rec = {"operator_setup_ms": round(setup_ms, 3)}
+    for name, prec, pc in (("fp32_refined", "fp32_refined", "none"), ("fp64", "fp64", "none"),
+                           ("fp32_refined_jacobi", "fp32_refined", "jacobi"), ("fp64_jacobi", "fp64", "jacobi")):
+        p = native.Problem(req, precision=prec, tables=tables, precond=pc)
+        p.solve(tol=1e-5)   # warm
+        info = p.solve(tol=1e-5)
+        rec[name] = {"solve_ms": round(info["ms"], 3), "iterations": info["iterations"],
+                     "field_rel_err": rel_err(p.field(), e_ref)}
+        del p
+    del p64
+    native.check(native.lib().tdda_trim_caches())
+    return rec
+
+
+def target_gradient_record(native):
+    """BASELINE c3: 48x48x24, d = 12 nm, 542 nm, x-polarised, seeded uniform
+    density through a linear cone filter of radius 2 voxels, mean |E|^2 over a
+    4x4x4 target block 60 nm above the structure.  ms per objective + adjoint
+    gradient through the torch.autograd.Function in FP32-refined (tol 1e-5)
+    and FP64 (tol 1e-5 and 1e-10), and each gradient against the tight FP64
+    one (the central-difference check on 32 voxels is a GPU test)."""
+    import torch
+    from dda_model.autograd import cone_filter, dda_objective, target_block
+    shape = (48, 48, 24)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(3)
+    raw = torch.tensor(rng.uniform(0, 1, (nz, ny, nx)), dtype=torch.float64, device="cuda", requires_grad=True)
+    tg = np.asarray(target_block(shape, 12.0, size=4, height_nm=60.0))
+    req = native.Request(shape=shape, d=12.0, lam=542.0, rho=np.full(nx * ny * nz, 0.5))
+    probs = {p: native.Problem(req, precision=p) for p in ("fp64", "fp32_refined")}
+    out, grads = {}, {}
+    for name, prec, tol in (("fp64_tight", "fp64", 1e-10), ("fp32_refined", "fp32_refined", 1e-5),
+                            ("fp64", "fp64", 1e-5)):
+        for rep in range(2):   # the first call is a warm-up
+            raw.grad = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            val = dda_objective(cone_filter(raw, 2.0).reshape(-1), probs[prec], kind="targets", targets=tg, tol=tol)
+            val.backward()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        grads[name] = raw.grad.detach().cpu().numpy().copy()
+        out[name] = {"ms": round(dt * 1e3, 2), "objective": val.item()}
+    for name in ("fp32_refined", "fp64"):
+        out[name]["gradient_rel_err"] = rel_err(grads[name], grads["fp64_tight"])
+    del probs
     native.check(native.lib().tdda_trim_caches())
     return out
 
@@ -535,6 +604,11 @@ def main():
         print(f"# multi-RHS 128x128x64: {c4['fp32_refined_jacobi']['solves_per_s']} solves/s refined", file=sys.stderr,
               flush=True)
 
+    import dda_model
+    pkg_tables = dda_model.sici_tables()   # the package's Si/Ci tables (FCD)
+    c1 = None if a.no_c4 else periodic_cell_record(native, pkg_tables)
+    c3 = None if a.no_c4 else target_gradient_record(native)
+
     cpu, lattice = None, None
     host = host_cpu()
     if not a.no_cpu:
@@ -553,7 +627,7 @@ def main():
                "single_thread": {"value": round(rate1, 5), "cores": 1, "sample_s": round(s1, 1),
                                  "detail": desc1},
                "host": host}
-        lattice = lattice_sum_rates(native, O.sici_tables(), int(os.environ.get("OMP_NUM_THREADS", threads)))
+        lattice = lattice_sum_rates(native, pkg_tables, int(os.environ.get("OMP_NUM_THREADS", threads)))
 
     iters = [r["iterations"] for r in res_all]
     out = {
@@ -602,6 +676,8 @@ def main():
         "server": {k: stats_sat[k] for k in ("chunks", "rebuilds", "launches", "max_active", "peak_bytes")},
         "copy_GBs": round(copy_gbs, 1),
         "multi_rhs_128x128x64": c4,
+        "periodic_cell_32x32x12": c1,
+        "target_gradient_48x48x24": c3,
         "lattice_sum": lattice,
         "roofline": roof,
         "cpu_baseline": cpu,
