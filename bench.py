@@ -519,9 +519,12 @@ def main():
                            precond=a.precond, ru_delta=a.ru_delta)
     next_id = 0
     # warmup: requests outside the timed seeds (plans, twiddles, allocator,
-    # system pool, code objects)
+    # system pool, code objects), as many as the server has slots so that its
+    # per-shape system pool and the device allocator hold what a full batch
+    # uses, as in a running service; every timed request still sets up its own
+    # operator and system
     for w in range(a.warmup):
-        wreqs = make_requests(native, min(128, R), seed0=900000 + 128 * w)
+        wreqs = make_requests(native, max(128, min(a.slots, R)), seed0=900000 + 1024 * w)
         run_stream(native, wreqs, "fp32_refined", a.slots, a.chunk, server=server, id0=next_id)
         next_id += len(wreqs)
     print("# warmup done", file=sys.stderr, flush=True)
