@@ -5,6 +5,7 @@
 
 #include <complex>
 #include <memory>
+#include <vector>
 
 #include "runtime.h"
 
@@ -50,6 +51,21 @@ class Problem {
     DevBuf<double> rho;
     SiciDevice tables;   // FCD tables (device), for field evaluations off the grid
     bool op_hit = false;
+};
+
+// set_density for a round of problems with one upload and one launch: the
+// densities (host) and a table of the problems' set-up parameters go through
+// a pinned stage into a device buffer, and one kernel sets every problem's
+// density, 1/alpha, incident field and efac and clears its solution vector.
+// The caller must not run a round on the same RoundSetup before the previous
+// round's kernel has completed (the stage and the device buffer are reused).
+class RoundSetup {
+   public:
+    void run(const std::vector<Problem*>& probs, const std::vector<const double*>& rho, hipStream_t st);
+
+   private:
+    PinnedStage stage_;
+    DevBuf<char> dev_;
 };
 
 }  // namespace tdda

@@ -2,6 +2,7 @@
 #include "problem.h"
 
 #include <cmath>
+#include <cstring>
 
 namespace tdda {
 
@@ -38,12 +39,12 @@ __device__ inline double2 inv_alpha(const SetupParams& P, double2 eps) {
     return cdiv_d(make_double2(1.0, 0.0), alpha);
 }
 
-__global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, double2* __restrict__ al64,
-                                float2* __restrict__ al32, double2* __restrict__ ial64, float2* __restrict__ ial32,
-                                double2* __restrict__ rhs, float2* __restrict__ efac) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const double2 eps = eps_of(P, rho[i]);
+// per-dipole set-up of dipole i (density value r)
+__device__ inline void setup_dipole(const SetupParams& P, int i, double r, double2* __restrict__ al64,
+                                    float2* __restrict__ al32, double2* __restrict__ ial64,
+                                    float2* __restrict__ ial32, double2* __restrict__ rhs,
+                                    float2* __restrict__ efac) {
+    const double2 eps = eps_of(P, r);
     const double2 al = inv_alpha(P, eps);
     al64[i] = al;
     al32[i] = make_float2((float)al.x, (float)al.y);
@@ -59,6 +60,44 @@ __global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, d
     double s, c;
     sincos(ph, &s, &c);
     for (int k = 0; k < 3; ++k) rhs[3 * i + k] = make_double2(P.e0 * P.ne0[k] * c, P.e0 * P.ne0[k] * s);
+}
+
+__global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, double2* __restrict__ al64,
+                                float2* __restrict__ al32, double2* __restrict__ ial64, float2* __restrict__ ial32,
+                                double2* __restrict__ rhs, float2* __restrict__ efac) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    setup_dipole(P, i, rho[i], al64, al32, ial64, ial32, rhs, efac);
+}
+
+// one problem of a round set-up; its blocks are [block0, block0 + ceil(n / 256))
+struct RoundJob {
+    SetupParams P;
+    long rho_off;   // byte offset of its densities in the round buffer
+    int block0;
+    double* rho;
+    double2 *al64, *ial64, *rhs, *sol;
+    float2 *al32, *ial32, *efac;
+};
+
+constexpr int ROUND_THREADS = 256;
+
+__global__ void __launch_bounds__(ROUND_THREADS)
+k_problem_setup_round(const RoundJob* __restrict__ jobs, int njobs, const char* __restrict__ base) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;   // last job with block0 <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const RoundJob& J = jobs[lo];
+    const int i = (b - J.block0) * ROUND_THREADS + threadIdx.x;
+    if (i >= J.P.n) return;
+    const double r = reinterpret_cast<const double*>(base + J.rho_off)[i];
+    J.rho[i] = r;
+    setup_dipole(J.P, i, r, J.al64, J.al32, J.ial64, J.ial32, J.rhs, J.efac);
+    for (int k = 0; k < 3; ++k) J.sol[3 * i + k] = make_double2(0.0, 0.0);
 }
 
 __global__ void k_problem_field(SetupParams P, const double* __restrict__ rho, const double2* __restrict__ al64,
@@ -145,6 +184,43 @@ void Problem::set_density(const double* r, bool on_device, hipStream_t st) {
                        sys->al32.get(), sys->ial64.get(), sys->ial32.get(), sys->rhs.get(), sys->efac.get());
     TDDA_HIP(hipGetLastError());
     sys->job.efac = sys->efac.get();
+}
+
+void RoundSetup::run(const std::vector<Problem*>& probs, const std::vector<const double*>& rho, hipStream_t st) {
+    TDDA_REQUIRE(probs.size() == rho.size(), ERR_INVALID, "one density per problem");
+    if (probs.empty()) return;
+    const int nj = (int)probs.size();
+    auto al256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = al256(nj * sizeof(RoundJob));
+    for (const Problem* p : probs) bytes += al256(p->ndip * sizeof(double));
+    char* h = stage_.reserve(bytes);
+    if (dev_.size() < bytes) dev_.alloc(std::max(bytes, 2 * dev_.size()));
+    RoundJob* tab = reinterpret_cast<RoundJob*>(h);
+    size_t off = al256(nj * sizeof(RoundJob));
+    int blocks = 0;
+    for (int j = 0; j < nj; ++j) {
+        Problem& p = *probs[j];
+        std::memcpy(h + off, rho[j], p.ndip * sizeof(double));
+        RoundJob& J = tab[j];
+        J.P = params_of(p);
+        J.rho_off = (long)off;
+        J.block0 = blocks;
+        J.rho = p.rho.get();
+        J.al64 = p.sys->al64.get();
+        J.al32 = p.sys->al32.get();
+        J.ial64 = p.sys->ial64.get();
+        J.ial32 = p.sys->ial32.get();
+        J.rhs = p.sys->rhs.get();
+        J.efac = p.sys->efac.get();
+        J.sol = p.sys->P.get();
+        p.sys->job.efac = p.sys->efac.get();
+        blocks += (p.ndip + ROUND_THREADS - 1) / ROUND_THREADS;
+        off += al256(p.ndip * sizeof(double));
+    }
+    TDDA_HIP(hipMemcpyAsync(dev_.get(), h, bytes, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_problem_setup_round, dim3(blocks), dim3(ROUND_THREADS), 0, st,
+                       reinterpret_cast<const RoundJob*>(dev_.get()), nj, dev_.get());
+    TDDA_HIP(hipGetLastError());
 }
 
 void Problem::field(double2* out, hipStream_t st) const {

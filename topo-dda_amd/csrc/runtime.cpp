@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <tuple>
 
 namespace tdda {
 
@@ -209,6 +210,32 @@ SiciDevice sici_tables(const double* si, const double* ci, long n, double delta)
 // ---------------------------------------------------------------------------
 // operators
 // ---------------------------------------------------------------------------
+namespace {
+// Timing events of operator set-ups, reused: the server creates and evicts
+// operators continuously, and event creation costs host time per request.
+std::mutex g_ev_mu;
+std::vector<hipEvent_t> g_ev_free;
+
+hipEvent_t take_event() {
+    {
+        std::lock_guard<std::mutex> lk(g_ev_mu);
+        if (!g_ev_free.empty()) {
+            hipEvent_t e = g_ev_free.back();
+            g_ev_free.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    TDDA_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void give_event(hipEvent_t e) {
+    std::lock_guard<std::mutex> lk(g_ev_mu);
+    g_ev_free.push_back(e);
+}
+}  // namespace
+
 bool OperatorKey::operator<(const OperatorKey& o) const {
     return std::tie(nx, ny, nz, d, K, method, maxm, maxn, lm, ln, sici_id) <
            std::tie(o.nx, o.ny, o.nz, o.d, o.K, o.method, o.maxm, o.maxn, o.lm, o.ln, o.sici_id);
@@ -237,8 +264,8 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     P.maxm = k.maxm; P.maxn = k.maxn; P.lm = k.lm; P.ln = k.ln;
     P.nsici = tables.n; P.delta = tables.delta;
 
-    TDDA_HIP(hipEventCreate(&start_));
-    TDDA_HIP(hipEventCreate(&done_));
+    start_ = take_event();
+    done_ = take_event();
     const long noct = (long)nx * ny * nz;
     tmp_.resize(4);
     tmp_[0].alloc(6 * noct);
@@ -260,8 +287,8 @@ Operator::~Operator() {
     // the temporaries and the spectrum may still be in use by the build
     if (done_) (void)hipEventSynchronize(done_);
     tmp_.clear();
-    if (start_) (void)hipEventDestroy(start_);
-    if (done_) (void)hipEventDestroy(done_);
+    if (start_) give_event(start_);
+    if (done_) give_event(done_);
 }
 
 double Operator::setup_ms() const {
@@ -533,7 +560,7 @@ char* PinnedStage::reserve(size_t bytes) {
     if (bytes > cap_) {
         if (p_) TDDA_HIP(hipHostFree(p_));
         p_ = nullptr;
-        const size_t c = std::max(bytes, 2 * cap_);
+        const size_t c = std::max({bytes, 2 * cap_, size_t(1) << 20});
         TDDA_HIP(hipHostMalloc((void**)&p_, c, hipHostMallocDefault));
         cap_ = c;
     }
@@ -550,6 +577,13 @@ void Batch::upload(DevBuf<T>& dst, const T* src, size_t n, size_t stage_off, hip
     char* stage = stage_.reserve(0) + stage_off;
     std::memcpy(stage, src, n * sizeof(T));
     dst.upload(reinterpret_cast<const T*>(stage), n, st);
+}
+
+// Batch buffers grow by half again their size at least: batches are rebuilt
+// every few chunks with a varying number of jobs, and each new size would
+// otherwise take a fresh device allocation.
+template <typename T> static void grow(DevBuf<T>& b, size_t n) {
+    if (b.size() < n) b.alloc(std::max<size_t>({n, b.size() + b.size() / 2, 1}));
 }
 
 void Batch::build(const std::vector<System*>& systems, KScalars* ks_dev, bool refined, hipStream_t st) {
@@ -575,7 +609,13 @@ namespace {
 // Z tiles of one job ordered so that tiles whose pencils are x/y mirrors of
 // each other (they read the same entries of the octant Green's spectrum) are
 // consecutive: sort by the mirrored centre pencil of the tile.
-std::vector<int> z_tile_order(const GridJob& J, int per_tile, int ntiles) {
+// The order depends only on (lx, ly, tile size, tile count); batches are
+// rebuilt every few chunks over the same few shapes, so each order is sorted
+// once per thread and kept.
+const std::vector<int>& z_tile_order(const GridJob& J, int per_tile, int ntiles) {
+    static thread_local std::map<std::tuple<int, int, int, int>, std::vector<int>> cache;
+    std::vector<int>& order = cache[std::make_tuple(J.lx, J.ly, per_tile, ntiles)];
+    if ((int)order.size() == ntiles) return order;
     std::vector<std::pair<long, int>> key(ntiles);
     for (int t = 0; t < ntiles; ++t) {
         const long pc = (long)t * per_tile + per_tile / 2;
@@ -585,7 +625,7 @@ std::vector<int> z_tile_order(const GridJob& J, int per_tile, int ntiles) {
     }
     std::stable_sort(key.begin(), key.end(),
                      [](const std::pair<long, int>& a, const std::pair<long, int>& b) { return a.first < b.first; });
-    std::vector<int> order(ntiles);
+    order.resize(ntiles);
     for (int t = 0; t < ntiles; ++t) order[t] = key[t].second;
     return order;
 }
@@ -707,7 +747,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             // contiguous and in job order
             for (size_t j = 0; j < hjobs.size(); ++j) {
                 const GridJob& J = hjobs[j];
-                const std::vector<int> order = z_tile_order(J, J.bz64, J.tz64);
+                const std::vector<int>& order = z_tile_order(J, J.bz64, J.tz64);
                 for (int t : order) m[1][PASS_Z][J.fz].push_back(make_int2((int)j, t));
             }
             continue;
@@ -717,7 +757,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
             if (pr == 0 && J0.tr32) continue;   // fused y/z pass (same shape for the whole group)
             const int T = pr == 0 ? J0.tz32 : J0.tz64;
             const int per = pr == 0 ? J0.bz32 : J0.bz64;
-            const std::vector<int> order = z_tile_order(J0, per, T);
+            const std::vector<int>& order = z_tile_order(J0, per, T);
             std::vector<int2>& mz = m[pr][PASS_Z][J0.fz];
             for (int t0 = 0; t0 < T; t0 += 8) {
                 const int t1 = std::min(T, t0 + 8);
@@ -764,9 +804,9 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     const size_t stage_maps = al256(hjobs.size() * sizeof(GridJob));
     const size_t stage_ranges = stage_maps + al256(total * sizeof(int2));
     if (pinned_) stage_.reserve(stage_ranges + al256((size_t)RU_REGIONS * (nj_all + 1) * sizeof(int)));
-    if (maps_.size() < total) maps_.alloc(std::max<size_t>(total, 1));
+    grow(maps_, total);
     upload(maps_, all.data(), total, stage_maps, st);
-    if (djobs_.size() < hjobs.size()) djobs_.alloc(std::max<size_t>(hjobs.size(), 1));
+    grow(djobs_, hjobs.size());
     upload_jobs(st);
     const int2* base = maps_.get();
     ls = LaunchSet{};
@@ -806,7 +846,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
     // pm[1][X][0] in build order
     if (refined) {
         const size_t n64 = off_u - off[1][0][0];
-        if (ru_maps_.size() < n64) ru_maps_.alloc(std::max<size_t>(n64, 1));
+        grow(ru_maps_, n64);
         if (ru_count_.size() < (size_t)RU_REGIONS) ru_count_.alloc(RU_REGIONS);
         ls.ru_map = ru_maps_.get();
         ls.ru_count = ru_count_.get();
@@ -828,7 +868,7 @@ void Batch::build(const std::vector<System*>& systems, const std::vector<KScalar
                     r[j] = (int)i;
                 }
             }
-        if (ru_ranges_.size() < rr.size()) ru_ranges_.alloc(rr.size());
+        grow(ru_ranges_, rr.size());
         upload(ru_ranges_, rr.data(), rr.size(), stage_ranges, st);
         ls.ru_ranges = ru_ranges_.get();
     }

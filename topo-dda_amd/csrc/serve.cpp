@@ -121,9 +121,16 @@ std::string Server::error() {
 // estimate fits; their operators and densities are set up on the setup stream.
 std::vector<int> Server::admit() {
     std::vector<int> joined;
+    // rounds grow with the live count (8, 8, 16, 32, ...): the first chunks
+    // start after a few set-ups instead of a full round of them
+    int limit;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        limit = std::max(1, std::min(opt_.admit_per_round, std::max(8, (int)busy_ / 2)));
+    }
     while (true) {
         ServeRequest req;
-        if ((int)joined.size() >= opt_.admit_per_round) break;
+        if ((int)joined.size() >= limit) break;
         {
             std::lock_guard<std::mutex> lk(mu_);
             if (pending_.empty()) break;
@@ -158,28 +165,27 @@ std::vector<int> Server::admit() {
             }
         }
         sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
-        sl.prob->sys->P.zero(setup_);
         sl.prob->sys->set_jacobi(opt_.jacobi);
         sl.op_hit = sl.prob->op_hit ? 1 : 0;
         sl.t_admit = now();
         sl.req = std::move(req);
     }
     if (joined.empty()) return joined;
-    // densities through this round's pinned stage (its previous round has
-    // uploaded once its event completed)
+    // densities and set-up of the whole round in one upload and one launch
+    // (its previous use has completed once its event has)
     const int r = rho_round_;
     rho_round_ ^= 1;
     if (rho_pending_[r]) TDDA_HIP(hipEventSynchronize(rho_done_[r]));
-    size_t total = 0;
-    for (int s : joined) total += slots_[s].req.rho.size();
-    double* stage = reinterpret_cast<double*>(rho_stage_[r].reserve(total * sizeof(double)));
+    std::vector<Problem*> probs;
+    std::vector<const double*> rho;
     for (int s : joined) {
-        Slot& sl = slots_[s];
-        std::memcpy(stage, sl.req.rho.data(), sl.req.rho.size() * sizeof(double));
-        sl.prob->set_density(stage, false, setup_);
-        stage += sl.req.rho.size();
-        sl.req.rho.clear();
-        sl.req.rho.shrink_to_fit();
+        probs.push_back(slots_[s].prob.get());
+        rho.push_back(slots_[s].req.rho.data());
+    }
+    round_setup_[r].run(probs, rho, setup_);
+    for (int s : joined) {
+        slots_[s].req.rho.clear();
+        slots_[s].req.rho.shrink_to_fit();
     }
     TDDA_HIP(hipEventRecord(rho_done_[r], setup_));
     rho_pending_[r] = true;

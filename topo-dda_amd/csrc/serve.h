@@ -32,7 +32,7 @@ struct ServerOptions {
     size_t hbm_budget = size_t(200) << 30;
     int max_slots = 64;
     int chunk = 4;                   // iterations between host checks
-    int admit_per_round = 32;        // admissions between two chunks (the first chunk starts early)
+    int admit_per_round = 32;        // largest admission round (rounds start at 8 and grow with the live count)
     bool keep_fields = true;         // keep E for take_field()
 };
 
@@ -162,7 +162,7 @@ class Server {
     // admission-thread state: densities of an admission round go up through
     // a pinned stage; two stages alternate, each reused after the round two
     // back has uploaded
-    PinnedStage rho_stage_[2];
+    RoundSetup round_setup_[2];   // density upload and set-up of a round (alternating)
     hipEvent_t rho_done_[2] = {nullptr, nullptr};
     bool rho_pending_[2] = {false, false};
     int rho_round_ = 0;
