@@ -220,6 +220,12 @@ typedef struct {
 /* precision: TDDA_PREC_*; tables may be null for LDR */
 int tdda_problem_create(const tdda_problem_desc* d, int precision, const double* si, const double* ci, long nsici,
                         double delta, tdda_problem** out);
+/* n problems at once: the Green's set-ups of their new operators (the
+ * AProductCore constructor's lattice sum and spectrum, src/AProductCore.cpp:
+ * 12-298, once per distinct operator) run as one launch per step for all of
+ * them instead of five launches per operator; out: n handles */
+int tdda_problems_create(const tdda_problem_desc* d, int n, int precision, const double* si, const double* ci,
+                         long nsici, double delta, tdda_problem** out);
 int tdda_problem_destroy(tdda_problem* p);
 /* rho: nx*ny*nz densities (clipped to [0,1]); on_device: rho is a device pointer */
 int tdda_problem_set_density(tdda_problem* p, const double* rho, int on_device);

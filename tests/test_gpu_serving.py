@@ -86,6 +86,36 @@ def test_batch_equals_alone(native, tables, precision, precond):
         assert _rel(p.field(), a) <= 1e-14
 
 
+def test_batched_green_setup_equals_direct(native, tables):
+    """Operators built together (one launch per set-up step for all of them,
+    as the server's admission rounds do) give the same spectra as operators
+    built one by one: the same matvecs, bitwise, with c64 and FP64
+    transforms.  The round holds two box shapes, a repeated operator (served
+    from the round itself) and an FCD box with images (lattice sum)."""
+    _log("[gpu] batched Green's set-up vs direct")
+    reqs = [_request(native, (16, 16, 16), seed=1, lam=431.0),
+            _request(native, (20, 12, 10), seed=2, lam=512.0, theta=0.2),
+            _request(native, (16, 16, 16), seed=3, lam=431.0),
+            _request(native, (12, 10, 6), seed=4, lam=601.0, method="FCD", images=2),
+            _request(native, (24, 24, 24), seed=5, lam=702.0, method="FCD")]
+    rng = np.random.default_rng(9)
+    xs = [rng.standard_normal(3 * int(np.prod(r.shape))) + 1j * rng.standard_normal(3 * int(np.prod(r.shape)))
+          for r in reqs]
+    native.trim_caches()
+    batched = native.Problem.create_many(reqs, precision="fp32_refined", tables=tables)
+    cached = [p.grid()[2] for p in batched]
+    assert cached == [False, False, True, False, False]
+    yb = {c64: native.apply(batched, xs, c64=c64) for c64 in (True, False)}
+    del batched
+    native.trim_caches()
+    direct = [native.Problem(r, precision="fp32_refined", tables=tables) for r in reqs]
+    assert [p.grid()[2] for p in direct] == [False, False, True, False, False]
+    for c64 in (True, False):
+        yd = native.apply(direct, xs, c64=c64)
+        for a, b in zip(yb[c64], yd):
+            assert np.array_equal(a, b)
+
+
 def test_server_mix_equals_alone_and_oracle(native, tables):
     import dda_oracle as O
     _log("[gpu] server mix vs alone")

@@ -369,6 +369,30 @@ int tdda_problem_create(const tdda_problem_desc* d, int precision, const double*
     });
 }
 
+int tdda_problems_create(const tdda_problem_desc* d, int n, int precision, const double* si, const double* ci,
+                         long nsici, double delta, tdda_problem** out) {
+    return guarded([&] {
+        TDDA_REQUIRE(out != nullptr && (d != nullptr || n == 0) && n >= 0, tdda::ERR_INVALID, "null argument");
+        TDDA_REQUIRE(precision == TDDA_PREC_FP64 || precision == TDDA_PREC_FP32_REFINED, tdda::ERR_INVALID,
+                     "unknown precision");
+        std::vector<std::unique_ptr<tdda::Problem>> made;
+        {
+            tdda::GreenBatch gb;   // cancelled by its destructor if a creation fails
+            for (int i = 0; i < n; ++i) {
+                tdda::ProblemDesc pd = to_desc(&d[i]);
+                tdda::SiciDevice t = tdda::sici_tables(pd.method == 0 ? si : nullptr, pd.method == 0 ? ci : nullptr,
+                                                       pd.method == 0 ? nsici : 0, delta);
+                made.emplace_back(new tdda::Problem(pd, t, precision == TDDA_PREC_FP32_REFINED,
+                                                    tdda::default_stream(), nullptr, nullptr, &gb));
+            }
+            gb.flush(tdda::default_stream());
+            // the batch's job tables are read by its launches
+            TDDA_HIP(hipStreamSynchronize(tdda::default_stream()));
+        }
+        for (int i = 0; i < n; ++i) out[i] = new tdda_problem{made[i].release()};
+    });
+}
+
 int tdda_problem_destroy(tdda_problem* p) {
     return guarded([&] {
         if (!p) return;

@@ -34,15 +34,19 @@ __device__ inline void image_term(const GreenParams& P, const double* si, const 
 }
 
 // One thread per octant offset (no images).
-__global__ void __launch_bounds__(256) k_green_single(GreenParams P, const double* __restrict__ si,
-                                                     const double* __restrict__ ci, double2* __restrict__ out) {
+__device__ inline void green_single_cell(const GreenParams& P, const double* __restrict__ si,
+                                         const double* __restrict__ ci, double2* __restrict__ out, long cell) {
     const long n = (long)P.nx * P.ny * P.nz;
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= n) return;
     const int ox = cell % P.nx, oy = (cell / P.nx) % P.ny, oz = cell / ((long)P.nx * P.ny);
     Sym6 t;
     image_term(P, si, ci, P.d * ox, P.d * oy, P.d * oz, 0, 0, t);
     for (int c = 0; c < 6; ++c) out[c * n + cell] = make_double2(t.re[c], t.im[c]);
+}
+
+__global__ void __launch_bounds__(256) k_green_single(GreenParams P, const double* __restrict__ si,
+                                                     const double* __restrict__ ci, double2* __restrict__ out) {
+    green_single_cell(P, si, ci, out, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // One workgroup per octant offset; threads split the (2MAXm+1)(2MAXn+1) images
@@ -138,13 +142,13 @@ __device__ inline void green_dft_column(const double2* __restrict__ src, long st
     for (int j = 0; j < KB; ++j) acc[j] = make_double2(2.0 * acc[j].x, 2.0 * acc[j].y);
 }
 
+// block b of one DFT axis
 template <int KB>
-__global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
-                                                   int outer, int no, int h, int inner, int L, int axis, int ng,
-                                                   const double2* __restrict__ tw) {
+__device__ inline void green_dft_block(const double2* __restrict__ in, double2* __restrict__ out, int outer, int no,
+                                       int h, int inner, int L, int axis, int ng, const double2* __restrict__ tw,
+                                       long b) {
     const long ncols = (long)outer * inner;
     const long nbc = (ncols + 255) / 256;
-    const long b = blockIdx.x;
     const long cb = b % nbc;
     const int g = (int)((b / nbc) % ng);
     const int c = (int)(b / (nbc * ng));
@@ -167,10 +171,16 @@ __global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ i
         if (j < nk) dst[(long)(k0 + j) * inner] = acc[j];
 }
 
-__global__ void k_green_interleave(const double2* __restrict__ in, long rows, int hx, double2* __restrict__ out64,
-                                   float2* __restrict__ out32) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;   // over [rows][6][hx]
-    if (i >= 6 * rows * hx) return;
+template <int KB>
+__global__ void __launch_bounds__(256) k_green_dft(const double2* __restrict__ in, double2* __restrict__ out,
+                                                   int outer, int no, int h, int inner, int L, int axis, int ng,
+                                                   const double2* __restrict__ tw) {
+    green_dft_block<KB>(in, out, outer, no, h, inner, L, axis, ng, tw, blockIdx.x);
+}
+
+__device__ inline void green_interleave_elem(const double2* __restrict__ in, long rows, int hx,
+                                             double2* __restrict__ out64, float2* __restrict__ out32, long i) {
+    if (i >= 6 * rows * hx) return;   // i over [rows][6][hx]
     const int x = (int)(i % hx);
     const long rc = i / hx;
     const long row = rc / 6;
@@ -178,6 +188,38 @@ __global__ void k_green_interleave(const double2* __restrict__ in, long rows, in
     const double2 v = in[(c * rows + row) * hx + x];
     out64[i] = v;
     out32[((row * 3 + (c >> 1)) * hx + x) * 2 + (c & 1)] = make_float2((float)v.x, (float)v.y);
+}
+
+__global__ void k_green_interleave(const double2* __restrict__ in, long rows, int hx, double2* __restrict__ out64,
+                                   float2* __restrict__ out32) {
+    green_interleave_elem(in, rows, hx, out64, out32, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the job of block b: the last with block0 <= b (jobs in increasing block0)
+template <typename Job> __device__ inline const Job& job_of_block(const Job* __restrict__ jobs, int nj, int b) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+__global__ void __launch_bounds__(256) k_green_single_batch(const GreenSingleJob* __restrict__ jobs, int nj) {
+    const GreenSingleJob& J = job_of_block(jobs, nj, blockIdx.x);
+    green_single_cell(J.P, J.si, J.ci, J.out, (long)(blockIdx.x - J.block0) * 256 + threadIdx.x);
+}
+
+template <int KB>
+__global__ void __launch_bounds__(256) k_green_dft_batch(const GreenDftJob* __restrict__ jobs, int nj) {
+    const GreenDftJob& J = job_of_block(jobs, nj, blockIdx.x);
+    green_dft_block<KB>(J.in, J.out, J.outer, J.no, J.h, J.inner, J.L, J.axis, J.ng, J.tw, blockIdx.x - J.block0);
+}
+
+__global__ void __launch_bounds__(256) k_green_interleave_batch(const GreenInterleaveJob* __restrict__ jobs, int nj) {
+    const GreenInterleaveJob& J = job_of_block(jobs, nj, blockIdx.x);
+    green_interleave_elem(J.in, J.rows, J.hx, J.out64, J.out32, (long)(blockIdx.x - J.block0) * 256 + threadIdx.x);
 }
 
 void launch_green_octant(const GreenParams& P, const double* si, const double* ci, double2* out, hipStream_t st) {
@@ -189,18 +231,48 @@ void launch_green_octant(const GreenParams& P, const double* si, const double* c
     TDDA_HIP(hipGetLastError());
 }
 
+// KB = 8 or 16 outputs per thread, whichever wastes fewer on the last group
+int green_dft_kb(int h) {
+    const int g8 = (h + 7) / 8, g16 = (h + 15) / 16;
+    return 8 * g8 < 16 * g16 ? 8 : 16;
+}
+
+int green_dft_blocks(int outer, int h, int inner) {
+    const int kb = green_dft_kb(h);
+    const long nbc = ((long)outer * inner + 255) / 256;
+    return (int)(6L * ((h + kb - 1) / kb) * nbc);
+}
+
 void launch_green_dft(const double2* in, double2* out, int outer, int no, int h, int inner, int L, int axis,
                       const double2* tw, hipStream_t st) {
-    // KB = 8 or 16 outputs per thread, whichever wastes fewer on the last group
-    const int g8 = (h + 7) / 8, g16 = (h + 15) / 16;
-    const bool small = 8 * g8 < 16 * g16;
-    const int ng = small ? g8 : g16;
-    const long nbc = ((long)outer * inner + 255) / 256;
-    const dim3 grid((unsigned)(6L * ng * nbc));
-    if (small)
+    const int kb = green_dft_kb(h);
+    const int ng = (h + kb - 1) / kb;
+    const dim3 grid((unsigned)green_dft_blocks(outer, h, inner));
+    if (kb == 8)
         hipLaunchKernelGGL(k_green_dft<8>, grid, dim3(256), 0, st, in, out, outer, no, h, inner, L, axis, ng, tw);
     else
         hipLaunchKernelGGL(k_green_dft<16>, grid, dim3(256), 0, st, in, out, outer, no, h, inner, L, axis, ng, tw);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_green_single_batch(const GreenSingleJob* jobs, int nj, int blocks, hipStream_t st) {
+    if (nj == 0) return;
+    hipLaunchKernelGGL(k_green_single_batch, dim3(blocks), dim3(256), 0, st, jobs, nj);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_green_dft_batch(const GreenDftJob* jobs, int nj, int blocks, int kb, hipStream_t st) {
+    if (nj == 0) return;
+    if (kb == 8)
+        hipLaunchKernelGGL(k_green_dft_batch<8>, dim3(blocks), dim3(256), 0, st, jobs, nj);
+    else
+        hipLaunchKernelGGL(k_green_dft_batch<16>, dim3(blocks), dim3(256), 0, st, jobs, nj);
+    TDDA_HIP(hipGetLastError());
+}
+
+void launch_green_interleave_batch(const GreenInterleaveJob* jobs, int nj, int blocks, hipStream_t st) {
+    if (nj == 0) return;
+    hipLaunchKernelGGL(k_green_interleave_batch, dim3(blocks), dim3(256), 0, st, jobs, nj);
     TDDA_HIP(hipGetLastError());
 }
 

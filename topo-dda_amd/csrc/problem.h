@@ -31,9 +31,12 @@ OperatorKey problem_operator_key(const ProblemDesc& d, const SiciDevice& tables)
 
 class Problem {
    public:
-    // op may be null: it is then taken from the operator cache (built if absent).
+    // op may be null: it is then taken from the operator cache (built if
+    // absent, through `batch` when one is given: the problem is usable once
+    // the batch has been flushed).
     Problem(const ProblemDesc& desc, const SiciDevice& tables, bool refined, hipStream_t st,
-            std::shared_ptr<Operator> op = nullptr, std::unique_ptr<System> reuse = nullptr);
+            std::shared_ptr<Operator> op = nullptr, std::unique_ptr<System> reuse = nullptr,
+            GreenBatch* batch = nullptr);
     // rho: N densities in [0, 1] (clipped), host or device memory; computes
     // 1/alpha (DDAModel.cpp:317-328, Tools.cpp:353-366) and the incident field
     // (DDAModel.cpp:312-316) on the device.

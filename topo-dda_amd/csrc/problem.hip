@@ -151,7 +151,7 @@ OperatorKey problem_operator_key(const ProblemDesc& d, const SiciDevice& tables)
 }
 
 Problem::Problem(const ProblemDesc& d, const SiciDevice& tables, bool refined, hipStream_t st,
-                 std::shared_ptr<Operator> op_, std::unique_ptr<System> reuse)
+                 std::shared_ptr<Operator> op_, std::unique_ptr<System> reuse, GreenBatch* batch)
     : desc(d), op(std::move(op_)), tables(tables) {
     TDDA_REQUIRE(d.nx >= 1 && d.ny >= 1 && d.nz >= 1, ERR_INVALID, "empty box");
     TDDA_REQUIRE(d.method == 0 || d.method == 1, ERR_METHOD, "method must be FCD (0) or LDR (1)");
@@ -159,7 +159,7 @@ Problem::Problem(const ProblemDesc& d, const SiciDevice& tables, bool refined, h
     K = 2.0 * M_PI / (d.lam / d.nback);
     problem_materials(d, &m0, &m1);
     ndip = d.nx * d.ny * d.nz;
-    if (!op) op = OperatorCache::instance().get(problem_operator_key(d, tables), tables, st, &op_hit);
+    if (!op) op = OperatorCache::instance().get(problem_operator_key(d, tables), tables, st, &op_hit, batch);
     else op_hit = true;
     if (reuse && reuse->job.lx == op->lx && reuse->job.ly == op->ly && reuse->job.lz == op->lz &&
         reuse->ndip == ndip && reuse->refined == refined && reuse->job.box_map == nullptr &&

@@ -241,7 +241,7 @@ bool OperatorKey::operator<(const OperatorKey& o) const {
            std::tie(o.nx, o.ny, o.nz, o.d, o.K, o.method, o.maxm, o.maxn, o.lm, o.ln, o.sici_id);
 }
 
-Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t st) : key(k) {
+Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t st, GreenBatch* batch) : key(k) {
     nx = k.nx; ny = k.ny; nz = k.nz;
     TDDA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, ERR_INVALID, "grid dimensions must be positive");
     lx = choose_fft_length(nx);
@@ -258,14 +258,14 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     if (k.method == 0)
         TDDA_REQUIRE(tables.si != nullptr && tables.n > 0, ERR_INVALID, "FCD needs Si/Ci tables");
 
-    GreenParams P{};
+    GreenParams& P = gp_;
     P.nx = nx; P.ny = ny; P.nz = nz;
     P.d = k.d; P.K = k.K; P.method = k.method;
     P.maxm = k.maxm; P.maxn = k.maxn; P.lm = k.lm; P.ln = k.ln;
     P.nsici = tables.n; P.delta = tables.delta;
+    si_ = tables.si;
+    ci_ = tables.ci;
 
-    start_ = take_event();
-    done_ = take_event();
     const long noct = (long)nx * ny * nz;
     tmp_.resize(4);
     tmp_[0].alloc(6 * noct);
@@ -274,8 +274,14 @@ Operator::Operator(const OperatorKey& k, const SiciDevice& tables, hipStream_t s
     tmp_[3].alloc(6L * hz * hy * hx);
     green64.alloc(6L * hz * hy * hx);
     green32.alloc(6L * hz * hy * hx);
+    if (batch) {   // the batch runs the other steps (its flush records done_)
+        if (P.maxm != 0 || P.maxn != 0) launch_green_octant(P, si_, ci_, tmp_[0].get(), st);
+        return;
+    }
+    start_ = take_event();
+    done_ = take_event();
     TDDA_HIP(hipEventRecord(start_, st));
-    launch_green_octant(P, tables.si, tables.ci, tmp_[0].get(), st);
+    launch_green_octant(P, si_, ci_, tmp_[0].get(), st);
     launch_green_dft(tmp_[0].get(), tmp_[1].get(), nz * ny, nx, hx, 1, lx, 0, tw.tw64 + px.tw_off, st);
     launch_green_dft(tmp_[1].get(), tmp_[2].get(), nz, ny, hy, hx, ly, 1, tw.tw64 + py.tw_off, st);
     launch_green_dft(tmp_[2].get(), tmp_[3].get(), 1, nz, hz, hy * hx, lz, 2, tw.tw64 + pz.tw_off, st);
@@ -292,10 +298,99 @@ Operator::~Operator() {
 }
 
 double Operator::setup_ms() const {
+    if (!start_) return -1.0;
     TDDA_HIP(hipEventSynchronize(done_));
     float ms = 0.f;
     TDDA_HIP(hipEventElapsedTime(&ms, start_, done_));
     return ms;
+}
+
+// ---------------------------------------------------------------------------
+// batched Green's set-ups
+// ---------------------------------------------------------------------------
+GreenBatch::~GreenBatch() { cancel(); }
+
+void GreenBatch::add(const std::shared_ptr<Operator>& op, hipStream_t st) {
+    ops_.push_back(op);
+    st_ = st;
+}
+
+std::shared_ptr<Operator> GreenBatch::find(const OperatorKey& key) const {
+    for (const auto& op : ops_)
+        if (!(op->key < key) && !(key < op->key)) return op;
+    return nullptr;
+}
+
+void GreenBatch::cancel() {
+    if (ops_.empty()) return;
+    if (st_) (void)hipStreamSynchronize(st_);   // queued lattice sums write the temporaries
+    OperatorCache::instance().publish(ops_, false);
+    ops_.clear();
+}
+
+void GreenBatch::flush(hipStream_t st) {
+    if (ops_.empty()) return;
+    const TwiddlePools tw = PlanRegistry::instance().pools();
+    std::vector<GreenSingleJob> single;
+    std::vector<GreenDftJob> dft[3][2];   // axis, 8 or 16 outputs per thread
+    std::vector<GreenInterleaveJob> inter;
+    int bs = 0, bi = 0, bd[3][2] = {};
+    for (const auto& sp : ops_) {
+        Operator& o = *sp;
+        if (o.gp_.maxm == 0 && o.gp_.maxn == 0) {
+            single.push_back(GreenSingleJob{o.gp_, o.si_, o.ci_, o.tmp_[0].get(), bs});
+            bs += green_single_blocks(o.gp_);
+        }
+        // the three axes of the direct build, in its order
+        const struct {
+            const double2* in;
+            double2* out;
+            int outer, no, h, inner, L;
+            const double2* tw;
+        } ax[3] = {{o.tmp_[0].get(), o.tmp_[1].get(), o.nz * o.ny, o.nx, o.hx, 1, o.lx, tw.tw64 + o.px.tw_off},
+                   {o.tmp_[1].get(), o.tmp_[2].get(), o.nz, o.ny, o.hy, o.hx, o.ly, tw.tw64 + o.py.tw_off},
+                   {o.tmp_[2].get(), o.tmp_[3].get(), 1, o.nz, o.hz, o.hy * o.hx, o.lz, tw.tw64 + o.pz.tw_off}};
+        for (int a = 0; a < 3; ++a) {
+            const int kb = green_dft_kb(ax[a].h), q = kb == 8 ? 0 : 1;
+            dft[a][q].push_back(GreenDftJob{ax[a].in, ax[a].out, ax[a].tw, ax[a].outer, ax[a].no, ax[a].h,
+                                            ax[a].inner, ax[a].L, a, (ax[a].h + kb - 1) / kb, bd[a][q]});
+            bd[a][q] += green_dft_blocks(ax[a].outer, ax[a].h, ax[a].inner);
+        }
+        inter.push_back(GreenInterleaveJob{o.tmp_[3].get(), o.green64.get(), o.green32.get(), (long)o.hz * o.hy,
+                                           o.hx, bi});
+        bi += green_interleave_blocks((long)o.hz * o.hy, o.hx);
+    }
+    // the job tables back to back (256-byte aligned) in one upload
+    auto al256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = al256(single.size() * sizeof(GreenSingleJob)) + al256(inter.size() * sizeof(GreenInterleaveJob));
+    for (auto& a : dft)
+        for (auto& v : a) bytes += al256(v.size() * sizeof(GreenDftJob));
+    char* h = stage_.reserve(bytes);
+    if (dev_.size() < bytes) dev_.alloc(std::max(bytes, 2 * dev_.size()));
+    size_t off = 0;
+    auto put = [&](const void* src, size_t n) {
+        if (n) std::memcpy(h + off, src, n);
+        const size_t at = off;
+        off += al256(n);
+        return dev_.get() + at;
+    };
+    const auto* d_single = reinterpret_cast<const GreenSingleJob*>(put(single.data(), single.size() * sizeof(GreenSingleJob)));
+    const GreenDftJob* d_dft[3][2];
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 2; ++q)
+            d_dft[a][q] = reinterpret_cast<const GreenDftJob*>(put(dft[a][q].data(), dft[a][q].size() * sizeof(GreenDftJob)));
+    const auto* d_inter = reinterpret_cast<const GreenInterleaveJob*>(put(inter.data(), inter.size() * sizeof(GreenInterleaveJob)));
+    TDDA_HIP(hipMemcpyAsync(dev_.get(), h, off, hipMemcpyHostToDevice, st));
+    launch_green_single_batch(d_single, (int)single.size(), bs, st);
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 2; ++q) launch_green_dft_batch(d_dft[a][q], (int)dft[a][q].size(), bd[a][q], q ? 16 : 8, st);
+    launch_green_interleave_batch(d_inter, (int)inter.size(), bi, st);
+    for (const auto& sp : ops_) {
+        sp->done_ = take_event();
+        TDDA_HIP(hipEventRecord(sp->done_, st));
+    }
+    OperatorCache::instance().publish(ops_, true);
+    ops_.clear();
 }
 
 bool Operator::reap() {
@@ -360,7 +455,14 @@ void OperatorCache::touch_locked(const OperatorKey& key) {
 // One thread builds a missing operator; others asking for the same key wait
 // for it instead of building a second copy.
 std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st,
-                                             bool* hit) {
+                                             bool* hit, GreenBatch* batch) {
+    if (batch) {   // queued earlier in the same batch: same stream, set up by the same flush
+        std::shared_ptr<Operator> q = batch->find(key);
+        if (q) {
+            if (hit) *hit = true;
+            return q;
+        }
+    }
     {
         std::unique_lock<std::mutex> lk(mu_);
         built_.wait(lk, [&] { return building_.count(key) == 0; });
@@ -378,7 +480,7 @@ std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciD
     if (hit) *hit = false;
     std::shared_ptr<Operator> op;
     try {
-        op = std::make_shared<Operator>(key, tables, st);
+        op = std::make_shared<Operator>(key, tables, st, batch);
     } catch (...) {
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -387,28 +489,47 @@ std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciD
         built_.notify_all();
         throw;
     }
+    if (batch) {   // stays "being built" until the batch's flush publishes it
+        batch->add(op, st);
+        return op;
+    }
     {
         std::lock_guard<std::mutex> lk(mu_);
         building_.erase(key);
-        map_[key] = op;
-        unreaped_.push_back(op);
-        touch_locked(key);
-        size_t total = 0;
-        for (auto& kv : map_) total += kv.second->bytes();
-        // evict least recently used entries nobody else holds
-        for (size_t i = 0; i < lru_.size() && total > budget_;) {
-            auto it = map_.find(lru_[i]);
-            if (it != map_.end() && it->second.use_count() == 1 && it->second != op) {
-                total -= it->second->bytes();
-                map_.erase(it);
-                lru_.erase(lru_.begin() + i);
-            } else {
-                ++i;
-            }
-        }
+        insert_locked(key, op);
     }
     built_.notify_all();
     return op;
+}
+
+void OperatorCache::insert_locked(const OperatorKey& key, const std::shared_ptr<Operator>& op) {
+    map_[key] = op;
+    unreaped_.push_back(op);
+    touch_locked(key);
+    size_t total = 0;
+    for (auto& kv : map_) total += kv.second->bytes();
+    // evict least recently used entries nobody else holds
+    for (size_t i = 0; i < lru_.size() && total > budget_;) {
+        auto it = map_.find(lru_[i]);
+        if (it != map_.end() && it->second.use_count() == 1 && it->second != op) {
+            total -= it->second->bytes();
+            map_.erase(it);
+            lru_.erase(lru_.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+void OperatorCache::publish(const std::vector<std::shared_ptr<Operator>>& ops, bool built) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (const auto& op : ops) {
+            building_.erase(op->key);
+            if (built) insert_locked(op->key, op);
+        }
+    }
+    built_.notify_all();
 }
 
 // ---------------------------------------------------------------------------
@@ -554,13 +675,18 @@ void System::get_solution(double2* h, hipStream_t st) const { P.download(h, 3ull
 // ---------------------------------------------------------------------------
 PinnedStage::~PinnedStage() {
     if (p_) (void)hipHostFree(p_);
+    for (char* q : old_) (void)hipHostFree(q);
 }
 
+// A grown stage keeps its earlier buffers until it is destroyed: freeing
+// pinned memory waits for the device, which would stall a host thread that
+// is preparing work while the GPU runs (the buffers add up to less than the
+// last one).
 char* PinnedStage::reserve(size_t bytes) {
     if (bytes > cap_) {
-        if (p_) TDDA_HIP(hipHostFree(p_));
+        if (p_) old_.push_back(p_);
         p_ = nullptr;
-        const size_t c = std::max({bytes, 2 * cap_, size_t(1) << 20});
+        const size_t c = std::max({bytes, 2 * cap_, size_t(4) << 20});
         TDDA_HIP(hipHostMalloc((void**)&p_, c, hipHostMallocDefault));
         cap_ = c;
     }

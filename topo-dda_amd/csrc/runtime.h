@@ -115,12 +115,32 @@ struct OperatorKey {
     bool operator<(const OperatorKey& o) const;
 };
 
+// Grow-only pinned host buffer: asynchronous uploads from it do not wait for
+// the stream (uploads from pageable memory block the host until the stream
+// reaches them).  The owner must not overwrite bytes an upload still reads.
+class PinnedStage {
+   public:
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage();
+    char* reserve(size_t bytes);   // contents are not kept on growth
+   private:
+    char* p_ = nullptr;
+    size_t cap_ = 0;
+    std::vector<char*> old_;
+};
+
+class GreenBatch;
+
 // The build is enqueued on the caller's stream and not waited for: work on
 // that stream sees the spectrum; other streams wait on ready() (the cache
 // does this on every hit).  The build's temporaries live until it is done.
+// With a GreenBatch the steps are queued in the batch instead (a lattice sum
+// with images is still launched at once) and launched by its flush().
 class Operator {
    public:
-    Operator(const OperatorKey& key, const SiciDevice& tables, hipStream_t st);
+    Operator(const OperatorKey& key, const SiciDevice& tables, hipStream_t st, GreenBatch* batch = nullptr);
     ~Operator();
     Operator(const Operator&) = delete;
     Operator& operator=(const Operator&) = delete;
@@ -132,20 +152,54 @@ class Operator {
     size_t bytes() const { return green64.bytes() + green32.bytes(); }
     hipEvent_t ready() const { return done_; }
     // GPU time of the build (lattice sum, DFTs, layout) in ms; waits for it
+    // (-1 for an operator built in a batch: its steps ran with the others')
     double setup_ms() const;
     // free the build temporaries once the build has run; true when none are left
     bool reap();
 
    private:
+    friend class GreenBatch;
+    GreenParams gp_{};
+    const double *si_ = nullptr, *ci_ = nullptr;
     hipEvent_t start_ = nullptr, done_ = nullptr;
     std::vector<DevBuf<double2>> tmp_;
+};
+
+// Green's set-ups of a round of new operators in one launch per step (octant
+// tensor, three DFT axes, layout) instead of five launches per operator.
+// Operators queued in the batch are published to the cache by flush(); until
+// then their keys count as being built, and the cache hands later requests
+// for the same key in the same batch the queued operator.  A batch is reused
+// round after round; the caller must not flush it again before the previous
+// flush's launches have completed (their job tables are overwritten).
+class GreenBatch {
+   public:
+    GreenBatch() = default;
+    GreenBatch(const GreenBatch&) = delete;
+    GreenBatch& operator=(const GreenBatch&) = delete;
+    ~GreenBatch();
+    void add(const std::shared_ptr<Operator>& op, hipStream_t st);
+    std::shared_ptr<Operator> find(const OperatorKey& key) const;
+    void flush(hipStream_t st);
+    // drop the queued operators without building them (after a failure):
+    // their keys are released and they never enter the cache
+    void cancel();
+    bool empty() const { return ops_.empty(); }
+
+   private:
+    std::vector<std::shared_ptr<Operator>> ops_;
+    hipStream_t st_ = nullptr;   // stream of the queued lattice sums
+    PinnedStage stage_;
+    DevBuf<char> dev_;
 };
 
 // LRU cache of operators bounded by bytes
 class OperatorCache {
    public:
     static OperatorCache& instance();
-    std::shared_ptr<Operator> get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st, bool* hit);
+    // batch: queue a missing operator's set-up there (published by its flush)
+    std::shared_ptr<Operator> get(const OperatorKey& key, const SiciDevice& tables, hipStream_t st, bool* hit,
+                                  GreenBatch* batch = nullptr);
     void set_budget(size_t bytes) { budget_ = bytes; }
     size_t bytes() const;
     void clear();
@@ -154,6 +208,11 @@ class OperatorCache {
     size_t trim(size_t bytes);
 
    private:
+    friend class GreenBatch;
+    // a batch's operators enter the cache (built: their ready() events are
+    // recorded) or, abandoned, give up their keys
+    void publish(const std::vector<std::shared_ptr<Operator>>& ops, bool built);
+    void insert_locked(const OperatorKey& key, const std::shared_ptr<Operator>& op);
     void touch_locked(const OperatorKey& key);   // move key to the most recent end of lru_
     void reap_locked();                          // free finished builds' temporaries
     mutable std::mutex mu_;
@@ -237,20 +296,6 @@ struct SolveResult {
     double err_est = 0.0;   // refined mode: estimated relative field error (0: none)
 };
 
-// Grow-only pinned host buffer: asynchronous uploads from it do not wait for
-// the stream (uploads from pageable memory block the host until the stream
-// reaches them).  The owner must not overwrite bytes an upload still reads.
-class PinnedStage {
-   public:
-    PinnedStage() = default;
-    PinnedStage(const PinnedStage&) = delete;
-    PinnedStage& operator=(const PinnedStage&) = delete;
-    ~PinnedStage();
-    char* reserve(size_t bytes);   // contents are not kept on growth
-   private:
-    char* p_ = nullptr;
-    size_t cap_ = 0;
-};
 
 // Device-resident batch: descriptors and (job, tile) maps for every pass.
 class Batch {

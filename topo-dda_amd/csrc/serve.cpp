@@ -128,6 +128,60 @@ std::vector<int> Server::admit() {
         std::lock_guard<std::mutex> lk(mu_);
         limit = std::max(1, std::min(opt_.admit_per_round, std::max(8, (int)busy_ / 2)));
     }
+    // this round's set-up buffers (their previous round has completed once its event has)
+    const int r = rho_round_;
+    if (rho_pending_[r]) {
+        TDDA_HIP(hipEventSynchronize(rho_done_[r]));
+        rho_pending_[r] = false;
+    }
+    GreenBatch& gb = green_batch_[r];
+    try {
+        admit_requests(limit, gb, joined);
+    } catch (...) {
+        gb.cancel();
+        throw;
+    }
+    if (joined.empty()) return joined;
+    rho_round_ ^= 1;
+    // Green's set-ups of the round's new operators, then densities and
+    // per-dipole set-up of every request, each in one upload and a few launches
+    gb.flush(setup_);
+    std::vector<Problem*> probs;
+    std::vector<const double*> rho;
+    for (int s : joined) {
+        probs.push_back(slots_[s].prob.get());
+        rho.push_back(slots_[s].req.rho.data());
+    }
+    round_setup_[r].run(probs, rho, setup_);
+    for (int s : joined) {
+        slots_[s].req.rho.clear();
+        slots_[s].req.rho.shrink_to_fit();
+    }
+    TDDA_HIP(hipEventRecord(rho_done_[r], setup_));
+    rho_pending_[r] = true;
+    // hand the round to the loop thread: it joins once this event completes
+    JoinRound jr;
+    jr.slots = joined;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!free_events_.empty()) {
+            jr.ev = free_events_.back();
+            free_events_.pop_back();
+        }
+    }
+    if (!jr.ev) TDDA_HIP(hipEventCreateWithFlags(&jr.ev, hipEventDisableTiming));
+    TDDA_HIP(hipEventRecord(jr.ev, setup_));
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        joining_.push_back(std::move(jr));
+    }
+    cv_.notify_all();
+    return joined;
+}
+
+// Take queued requests into free slots (up to `limit`) and create their
+// problems, queuing new operators' set-ups in gb.
+void Server::admit_requests(int limit, GreenBatch& gb, std::vector<int>& joined) {
     while (true) {
         ServeRequest req;
         if ((int)joined.size() >= limit) break;
@@ -164,49 +218,12 @@ std::vector<int> Server::admit() {
                 pit->second.pop_back();
             }
         }
-        sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse)));
+        sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse), &gb));
         sl.prob->sys->set_jacobi(opt_.jacobi);
         sl.op_hit = sl.prob->op_hit ? 1 : 0;
         sl.t_admit = now();
         sl.req = std::move(req);
     }
-    if (joined.empty()) return joined;
-    // densities and set-up of the whole round in one upload and one launch
-    // (its previous use has completed once its event has)
-    const int r = rho_round_;
-    rho_round_ ^= 1;
-    if (rho_pending_[r]) TDDA_HIP(hipEventSynchronize(rho_done_[r]));
-    std::vector<Problem*> probs;
-    std::vector<const double*> rho;
-    for (int s : joined) {
-        probs.push_back(slots_[s].prob.get());
-        rho.push_back(slots_[s].req.rho.data());
-    }
-    round_setup_[r].run(probs, rho, setup_);
-    for (int s : joined) {
-        slots_[s].req.rho.clear();
-        slots_[s].req.rho.shrink_to_fit();
-    }
-    TDDA_HIP(hipEventRecord(rho_done_[r], setup_));
-    rho_pending_[r] = true;
-    // hand the round to the loop thread: it joins once this event completes
-    JoinRound jr;
-    jr.slots = joined;
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        if (!free_events_.empty()) {
-            jr.ev = free_events_.back();
-            free_events_.pop_back();
-        }
-    }
-    if (!jr.ev) TDDA_HIP(hipEventCreateWithFlags(&jr.ev, hipEventDisableTiming));
-    TDDA_HIP(hipEventRecord(jr.ev, setup_));
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        joining_.push_back(std::move(jr));
-    }
-    cv_.notify_all();
-    return joined;
 }
 
 // Admission thread: admit whenever requests wait and a slot is free; when

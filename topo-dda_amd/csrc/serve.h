@@ -90,6 +90,7 @@ class Server {
     void loop();
     void admit_loop();
     std::vector<int> admit();
+    void admit_requests(int limit, GreenBatch& gb, std::vector<int>& joined);
     void fail(const std::string& what);
     void retire(const std::vector<int>& slots, const KScalars* snap);
     // finished requests whose field is still being copied to the host
@@ -163,6 +164,7 @@ class Server {
     // a pinned stage; two stages alternate, each reused after the round two
     // back has uploaded
     RoundSetup round_setup_[2];   // density upload and set-up of a round (alternating)
+    GreenBatch green_batch_[2];   // Green's set-ups of a round's new operators (alternating)
     hipEvent_t rho_done_[2] = {nullptr, nullptr};
     bool rho_pending_[2] = {false, false};
     int rho_round_ = 0;

@@ -65,6 +65,8 @@ def lib():
         L.tdda_last_error.restype = ctypes.c_char_p
         L.tdda_problem_create.argtypes = [ctypes.POINTER(ProblemDesc), ctypes.c_int, dp, dp, ctypes.c_long,
                                           ctypes.c_double, ctypes.POINTER(vp)]
+        L.tdda_problems_create.argtypes = [ctypes.POINTER(ProblemDesc), ctypes.c_int, ctypes.c_int, dp, dp,
+                                           ctypes.c_long, ctypes.c_double, ctypes.POINTER(vp)]
         L.tdda_problem_destroy.argtypes = [vp]
         L.tdda_problem_set_density.argtypes = [vp, ctypes.c_void_p, ctypes.c_int]
         L.tdda_problems_solve.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -165,6 +167,28 @@ class Problem:
         check(lib().tdda_problem_set_preconditioner(self.h, PRECOND[precond]))
         self.set_density(req.rho)
 
+    @classmethod
+    def create_many(cls, reqs, precision="fp32_refined", tables=None, precond="none"):
+        """Several problems at once: the Green's set-ups of their new operators
+        run together, one launch per step for all of them."""
+        si, ci, n, delta = _tables(tables)
+        k = len(reqs)
+        descs = (ProblemDesc * max(k, 1))(*[r.desc() for r in reqs])
+        hs = (ctypes.c_void_p * max(k, 1))()
+        check(lib().tdda_problems_create(descs, k, PREC[precision], _dp(si) if si is not None else None,
+                                         _dp(ci) if ci is not None else None, n, delta, hs))
+        out = []
+        for r, h in zip(reqs, hs[:k]):
+            p = cls.__new__(cls)
+            p._keep = (si, ci)
+            p.h = ctypes.c_void_p(h)
+            p.n = int(np.prod(r.shape))
+            out.append(p)
+        for r, p in zip(reqs, out):
+            check(lib().tdda_problem_set_preconditioner(p.h, PRECOND[precond]))
+            p.set_density(r.rho)
+        return out
+
     def set_density(self, rho):
         """rho: numpy array (host) or a torch CUDA tensor of float64 (device)."""
         if hasattr(rho, "data_ptr"):
@@ -261,6 +285,11 @@ def apply(problems, xs, c64=False):
         out.append(y[k:k + 3 * p.n].copy())
         k += 3 * p.n
     return out
+
+
+def trim_caches():
+    """Drop cached operators nobody holds and release cached device blocks."""
+    check(lib().tdda_trim_caches())
 
 
 def set_graph_capture(on: bool):
