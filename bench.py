@@ -229,15 +229,23 @@ def probe_shapes():
 
 def roofline_probe(native, reps, iteration=True):
     """One batched matvec (one launch per pass and class) over a serving-like
-    batch (48 problems of every shape, about what the server's 256 slots hold, distinct
-    wavelengths so no two share a Green's spectrum, c64), optionally the ten
-    launches of one refined BiCGSTAB step, and a device copy of known size for
+    batch (48 problems of every shape, about half of one step's 512 requests,
+    distinct wavelengths so no two share a Green's spectrum, c64), optionally the ten
+    launches of one refined BiCGSTAB step and the same matvec with separate
+    x and y passes (returned with them), and a device copy of known size for
     calibrating byte counters."""
     import torch
     probs = [native.Problem(native.Request(shape=s, d=15.0, lam=500.0 + 1.0 * i, rho=np.full(int(np.prod(s)), 0.5)),
                             precision="fp32_refined") for i, s in enumerate(probe_shapes())]
     ms5 = native.time_passes(probs, c64=True, reps=reps)
     ms10 = native.time_iteration(probs, reps=reps) if iteration else None
+    if iteration:   # the same matvec without the plane passes (separate x and y passes), for comparison
+        native.set_plane_passes(False)
+        try:
+            ms5_sep = native.time_passes(probs, c64=True, reps=reps)
+        finally:
+            native.set_plane_passes(True)
+        ms10 = (ms10, ms5_sep)
     src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")   # 256 MiB
     dst = torch.empty_like(src)
     dst.copy_(src)
@@ -454,7 +462,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--fp64-requests", type=int, default=96)
-    ap.add_argument("--slots", type=int, default=256)
+    ap.add_argument("--slots", type=int, default=512)
     ap.add_argument("--chunk", type=int, default=2)
     ap.add_argument("--load", type=float, default=0.8)
     ap.add_argument("--ru-delta", type=float, default=0.01,
@@ -595,9 +603,12 @@ def main():
     w_np, res_np, _, _ = run_stream(native, reqs0, "fp32_refined", a.slots, a.chunk, precond="none")
 
     # roofline of the batched matvec (one launch per pass over the probe batch)
-    ms5, ms10, probs = roofline_probe(native, 30)
+    ms5, (ms10, ms5_sep), probs = roofline_probe(native, 30)
     roof = roofline_record(ms5)
     roof["iteration_pass_ms"] = [round(float(x), 5) for x in ms10]
+    roof["separate_passes"] = {"matvec_ms": round(float(sum(ms5_sep)), 5),
+                               "pass_ms": [round(float(x), 5) for x in ms5_sep],
+                               "plane_passes_time_ratio": round(float(sum(ms5)) / float(sum(ms5_sep)), 4)}
     del probs
     native.check(native.lib().tdda_trim_caches())
     copy_gbs = copy_bandwidth()
