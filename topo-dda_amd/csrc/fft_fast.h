@@ -551,8 +551,9 @@ TDDA_HD bool with_split_z(int L, F&& f) {
         case 40: f(IC<10>{}, IC<4>{}); return true;
         case 48: f(IC<12>{}, IC<4>{}); return true;
         case 64: f(IC<16>{}, IC<4>{}); return true;
-        // 256 keeps the generic Z pass: a 16 x 16 tile of three components
-        // needs about 100 KB of LDS per block
+        // 256 keeps the generic Z pass under either variant: a 16 x 16 tile
+        // of three components needs about 100 KB of LDS per block, and its
+        // registers would set those of every other split of its kernel class
         case 256: return false;
         default: return with_split(L, f);
     }
@@ -568,7 +569,7 @@ TDDA_HD bool with_split_z(int L, F&& f) {
 template <int PASS, typename F>
 TDDA_HD bool with_split_pass(int L, int v, F&& f) {
     if constexpr (PASS == 3) return v ? with_split(L, f) : with_split_xy(L, f);
-    else if constexpr (PASS == 2) return v ? with_split(L, f) : with_split_z(L, f);
+    else if constexpr (PASS == 2) return v ? (L != 256 && with_split(L, f)) : with_split_z(L, f);
     else return v ? with_split_xy(L, f) : with_split(L, f);
 }
 
@@ -580,7 +581,7 @@ inline int split_variants(int pass, int L) {
     bool ha = false, hb = false;
     switch (pass) {
         case 3: ha = with_split_xy(L, ga); hb = with_split(L, gb); break;
-        case 2: ha = with_split_z(L, ga); hb = with_split(L, gb); break;
+        case 2: ha = with_split_z(L, ga); hb = L != 256 && with_split(L, gb); break;
         default: ha = with_split(L, ga); hb = with_split_xy(L, gb); break;
     }
     return (ha && hb && (a1 != b1 || a2 != b2)) ? 2 : 1;
