@@ -48,6 +48,24 @@ __host__ __device__ inline double2 ddiv(double2 a, double2 b) {
     return make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
 }
 
+#if defined(__HIPCC__)
+// Incident plane wave at cell (gx, gy, gz) (DDAModel.cpp:312-316): the
+// phase factor exp(i kd (nk . g)) with kd = K d, and a component
+// en exp(i ...) with en = e0 n_e0[c].  Every product and sum is rounded on
+// its own (no contraction), so each kernel that evaluates them gets the same
+// bits.
+__device__ inline double2 incident_phase(double kd, const double (&nk)[3], int gx, int gy, int gz) {
+    const double q = __dadd_rn(__dadd_rn(__dmul_rn(nk[0], (double)gx), __dmul_rn(nk[1], (double)gy)),
+                               __dmul_rn(nk[2], (double)gz));
+    double s, c;
+    sincos(__dmul_rn(kd, q), &s, &c);
+    return make_double2(c, s);
+}
+__device__ inline double2 incident_wave(double2 phase, double en) {
+    return make_double2(__dmul_rn(en, phase.x), __dmul_rn(en, phase.y));
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // error handling (host)
 // ---------------------------------------------------------------------------

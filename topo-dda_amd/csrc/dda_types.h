@@ -122,6 +122,14 @@ struct GridJob {
     // (||f dP||^2, ||f P||^2 per tile), apart from the X inverse partials
     double* partial_xf;        // [X forward tiles][2]
     const float2* efac;        // [ndip] local-field factor f = al 3/(eps + 2); null: f = 1
+    // r0_inc = 1: the shadow residual r0 of BiCGSTAB is the incident field
+    // (a solve from P = 0 whose right-hand side is that field, full box): the
+    // X inverse epilogues evaluate it (incident_phase / incident_wave with
+    // K d, n_k, e0 n_e0) instead of loading r0
+    int r0_inc;
+    double inc_kd;
+    double inc_nk[3];
+    double inc_en[3];
 };
 
 }  // namespace tdda

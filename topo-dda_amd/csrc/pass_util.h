@@ -344,6 +344,38 @@ __device__ inline int vec_index(const GridJob& J, int q, int x, int c) {
 // ----------------------------------------------------------------------------
 constexpr int XI_NACC = 8;
 
+// BiCGSTAB's shadow residual r0 at vector index i = 3 cell + c: loaded, or
+// the incident field when the job says so (full box)
+__device__ inline void cell_of(const GridJob& J, unsigned cell, int& gx, int& gy, int& gz) {
+    const unsigned row = cell / (unsigned)J.nx;
+    gx = (int)(cell - row * (unsigned)J.nx);
+    gy = (int)(row % (unsigned)J.ny);
+    gz = (int)(row / (unsigned)J.ny);
+}
+template <typename KC>
+__device__ inline KC r0_at(const GridJob& J, unsigned i) {
+    if (!J.r0_inc) return gload<KC>(J.r0, i);
+    const unsigned cell = i / 3u;
+    int gx, gy, gz;
+    cell_of(J, cell, gx, gy, gz);
+    return from_d<KC>(incident_wave(incident_phase(J.inc_kd, J.inc_nk, gx, gy, gz), J.inc_en[i - 3u * cell]));
+}
+// the three components at cell i / 3 (i a multiple of 3)
+template <typename KC>
+__device__ inline void r0_cell(const GridJob& J, unsigned i, KC (&rz)[3]) {
+    if (!J.r0_inc) {
+        static_for<3>([&](auto ci) { rz[decltype(ci)::value] = gload<KC>(J.r0, i + decltype(ci)::value); });
+        return;
+    }
+    int gx, gy, gz;
+    cell_of(J, i / 3u, gx, gy, gz);
+    const double2 ph = incident_phase(J.inc_kd, J.inc_nk, gx, gy, gz);
+    static_for<3>([&](auto ci) {
+        constexpr int c = decltype(ci)::value;
+        rz[c] = from_d<KC>(incident_wave(ph, J.inc_en[c]));
+    });
+}
+
 template <typename C, typename KC, int MODE>
 __device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __restrict__ al,
                                   double (&acc)[XI_NACC]) {
@@ -357,7 +389,7 @@ __device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __r
         const C y = cadd(conv, cmul(a, from_d<C>(to_d(gload<KC>(J.p, i)))));
         const KC vk = from_d<KC>(to_d(y));
         gstore<KC>(J.v, i, vk);
-        const double2 d = cmulc(to_d(vk), to_d(gload<KC>(J.r0, i)));   // conj(r0) v
+        const double2 d = cmulc(to_d(vk), to_d(r0_at<KC>(J, i)));   // conj(r0) v
         acc[0] += d.x;
         acc[1] += d.y;
     } else if (MODE == XI_S) {
@@ -366,7 +398,7 @@ __device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __r
         const KC sk = from_d<KC>(to_d(y));
         gstore<KC>(J.s, i, sk);
         const double2 sd = to_d(sk), td = to_d(tk);
-        const double2 rz = to_d(gload<KC>(J.r0, i));
+        const double2 rz = to_d(r0_at<KC>(J, i));
         const double2 d = cmulc(td, sd);    // conj(s) t
         const double2 e = cmulc(td, rz);    // conj(r0) t
         const double2 f = cmulc(sd, rz);    // conj(r0) s
@@ -383,7 +415,7 @@ __device__ inline void xi_element(const GridJob& J, int vi, C conv, const C* __r
         const double2 rr = csub(gload<double2>(J.rhs, i), y);
         const KC rk = from_d<KC>(rr);
         gstore<KC>(J.r, i, rk);
-        const double2 d = cmulc(to_d(rk), to_d(gload<KC>(J.r0, i)));
+        const double2 d = cmulc(to_d(rk), to_d(r0_at<KC>(J, i)));
         acc[0] += d.x;
         acc[1] += d.y;
         acc[2] += rr.x * rr.x + rr.y * rr.y;

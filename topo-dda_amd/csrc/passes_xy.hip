@@ -144,11 +144,8 @@ __device__ inline void xi_dipole(const GridJob& J, unsigned i, const float2 (&co
         });
     } else if constexpr (MODE == XI_V) {
         C pv[3], rz[3];
-        static_for<3>([&](auto ci) {
-            constexpr int c = decltype(ci)::value;
-            pv[c] = gload<C>(J.p, i + c);
-            rz[c] = gload<C>(J.r0, i + c);
-        });
+        static_for<3>([&](auto ci) { pv[decltype(ci)::value] = gload<C>(J.p, i + decltype(ci)::value); });
+        r0_cell(J, i, rz);
         static_for<3>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             const C vk = cadd(conv[c], cmul(a, pv[c]));
@@ -159,11 +156,8 @@ __device__ inline void xi_dipole(const GridJob& J, unsigned i, const float2 (&co
         });
     } else {   // XI_S
         C tv[3], rz[3];
-        static_for<3>([&](auto ci) {
-            constexpr int c = decltype(ci)::value;
-            tv[c] = gload<C>(J.t, i + c);
-            rz[c] = gload<C>(J.r0, i + c);
-        });
+        static_for<3>([&](auto ci) { tv[decltype(ci)::value] = gload<C>(J.t, i + decltype(ci)::value); });
+        r0_cell(J, i, rz);
         static_for<3>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             const C sk = cadd(conv[c], cmul(a, tv[c]));

@@ -41,6 +41,12 @@ class Problem {
     // 1/alpha (DDAModel.cpp:317-328, Tools.cpp:353-366) and the incident field
     // (DDAModel.cpp:312-316) on the device.
     void set_density(const double* rho, bool on_device, hipStream_t st);
+    // Later solves from P = 0 with the incident field as right-hand side (as
+    // the server runs them) take that field as BiCGSTAB's shadow residual r0
+    // (the initial residual, DDAModel.cpp:928) and evaluate it where it is
+    // used instead of loading it.  Not for solves with another right-hand
+    // side or a starting guess.
+    void use_incident_r0();
     // E = al P 3/(eps + 2) (DDAModel.cpp:1185-1192) into out (device, 3N complex)
     void field(double2* out, hipStream_t st) const;
     static size_t estimate_bytes(const ProblemDesc& d, bool refined, bool need_operator);

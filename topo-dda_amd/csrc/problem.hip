@@ -13,7 +13,9 @@ struct SetupParams {
     double d, K;
     double nk[3], ne0[3], e0;
     double2 m0, m1;
-    double S;   // sum_j (e_j k_j)^2
+    double S;        // sum_j (e_j k_j)^2
+    double kd;       // K d
+    double en[3];    // e0 n_e0
 };
 
 __device__ inline double2 cdiv_d(double2 a, double2 b) { return ddiv(a, b); }
@@ -56,10 +58,8 @@ __device__ inline void setup_dipole(const SetupParams& P, int i, double r, doubl
         efac[i] = make_float2((float)f.x, (float)f.y);
     }
     const int gx = i % P.nx, gy = (i / P.nx) % P.ny, gz = i / (P.nx * P.ny);
-    const double ph = P.K * P.d * (P.nk[0] * gx + P.nk[1] * gy + P.nk[2] * gz);
-    double s, c;
-    sincos(ph, &s, &c);
-    for (int k = 0; k < 3; ++k) rhs[3 * i + k] = make_double2(P.e0 * P.ne0[k] * c, P.e0 * P.ne0[k] * s);
+    const double2 ph = incident_phase(P.kd, P.nk, gx, gy, gz);
+    for (int k = 0; k < 3; ++k) rhs[3 * i + k] = incident_wave(ph, P.en[k]);
 }
 
 __global__ void k_problem_setup(SetupParams P, const double* __restrict__ rho, double2* __restrict__ al64,
@@ -127,6 +127,8 @@ SetupParams params_of(const Problem& p) {
     P.m1 = p.m1;
     P.S = 0.0;
     for (int k = 0; k < 3; ++k) P.S += (p.desc.ne0[k] * p.desc.nk[k]) * (p.desc.ne0[k] * p.desc.nk[k]);
+    P.kd = p.K * p.desc.d;
+    for (int k = 0; k < 3; ++k) P.en[k] = p.desc.e0 * p.desc.ne0[k];
     return P;
 }
 
@@ -221,6 +223,18 @@ void RoundSetup::run(const std::vector<Problem*>& probs, const std::vector<const
     hipLaunchKernelGGL(k_problem_setup_round, dim3(blocks), dim3(ROUND_THREADS), 0, st,
                        reinterpret_cast<const RoundJob*>(dev_.get()), nj, dev_.get());
     TDDA_HIP(hipGetLastError());
+}
+
+void Problem::use_incident_r0() {
+    TDDA_REQUIRE(sys->job.box_map == nullptr, ERR_INVALID, "the incident shadow residual needs the full box");
+    const SetupParams P = params_of(*this);
+    GridJob& J = sys->job;
+    J.r0_inc = 1;
+    J.inc_kd = P.kd;
+    for (int k = 0; k < 3; ++k) {
+        J.inc_nk[k] = P.nk[k];
+        J.inc_en[k] = P.en[k];
+    }
 }
 
 void Problem::field(double2* out, hipStream_t st) const {

@@ -219,6 +219,7 @@ void Server::admit_requests(int limit, GreenBatch& gb, std::vector<int>& joined)
             }
         }
         sl.prob.reset(new Problem(req.desc, tables_, opt_.refined, setup_, nullptr, std::move(reuse), &gb));
+        sl.prob->use_incident_r0();   // every solve starts from P = 0 with the incident field
         sl.prob->sys->set_jacobi(opt_.jacobi);
         sl.op_hit = sl.prob->op_hit ? 1 : 0;
         sl.t_admit = now();
