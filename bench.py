@@ -670,6 +670,8 @@ def main():
         "poisson_solves_per_s": round(R / w_p, 3),
         "converged": conv,
         "iterations_p50": percentile(iters, 0.5),
+        "reliable_updates_p50": percentile([r["reliable_updates"] for r in res_all], 0.5),
+        "reliable_updates_max": max(r["reliable_updates"] for r in res_all),
         "err_est_p50": percentile([r["err_est"] for r in res_all], 0.5),
         "err_est_max": max(r["err_est"] for r in res_all),
         "precond": a.precond,

@@ -308,7 +308,13 @@ double Operator::setup_ms() const {
 // ---------------------------------------------------------------------------
 // batched Green's set-ups
 // ---------------------------------------------------------------------------
-GreenBatch::~GreenBatch() { cancel(); }
+GreenBatch::~GreenBatch() {
+    cancel();
+    if (flushed_) {
+        (void)hipEventSynchronize(flushed_);
+        (void)hipEventDestroy(flushed_);
+    }
+}
 
 void GreenBatch::add(const std::shared_ptr<Operator>& op, hipStream_t st) {
     ops_.push_back(op);
@@ -365,6 +371,7 @@ void GreenBatch::flush(hipStream_t st) {
     size_t bytes = al256(single.size() * sizeof(GreenSingleJob)) + al256(inter.size() * sizeof(GreenInterleaveJob));
     for (auto& a : dft)
         for (auto& v : a) bytes += al256(v.size() * sizeof(GreenDftJob));
+    if (flushed_) TDDA_HIP(hipEventSynchronize(flushed_));   // the tables below are reused
     char* h = stage_.reserve(bytes);
     if (dev_.size() < bytes) dev_.alloc(std::max(bytes, 2 * dev_.size()));
     size_t off = 0;
@@ -385,6 +392,8 @@ void GreenBatch::flush(hipStream_t st) {
     for (int a = 0; a < 3; ++a)
         for (int q = 0; q < 2; ++q) launch_green_dft_batch(d_dft[a][q], (int)dft[a][q].size(), bd[a][q], q ? 16 : 8, st);
     launch_green_interleave_batch(d_inter, (int)inter.size(), bi, st);
+    if (!flushed_) TDDA_HIP(hipEventCreateWithFlags(&flushed_, hipEventDisableTiming));
+    TDDA_HIP(hipEventRecord(flushed_, st));
     for (const auto& sp : ops_) {
         sp->done_ = take_event();
         TDDA_HIP(hipEventRecord(sp->done_, st));
@@ -465,6 +474,14 @@ std::shared_ptr<Operator> OperatorCache::get(const OperatorKey& key, const SiciD
     }
     {
         std::unique_lock<std::mutex> lk(mu_);
+        if (batch && !batch->empty() && building_.count(key)) {
+            // another thread builds this key: publish this batch's queued
+            // builds before waiting, so that two batching threads never wait
+            // for each other's keys
+            lk.unlock();
+            batch->flush(st);
+            lk.lock();
+        }
         built_.wait(lk, [&] { return building_.count(key) == 0; });
         reap_locked();
         auto it = map_.find(key);

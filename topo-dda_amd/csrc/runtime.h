@@ -170,8 +170,8 @@ class Operator {
 // Operators queued in the batch are published to the cache by flush(); until
 // then their keys count as being built, and the cache hands later requests
 // for the same key in the same batch the queued operator.  A batch is reused
-// round after round; the caller must not flush it again before the previous
-// flush's launches have completed (their job tables are overwritten).
+// round after round (a flush first waits for the previous flush's launches,
+// which read the job tables it overwrites).
 class GreenBatch {
    public:
     GreenBatch() = default;
@@ -191,6 +191,7 @@ class GreenBatch {
     hipStream_t st_ = nullptr;   // stream of the queued lattice sums
     PinnedStage stage_;
     DevBuf<char> dev_;
+    hipEvent_t flushed_ = nullptr;   // recorded after the last flush's launches
 };
 
 // LRU cache of operators bounded by bytes
