@@ -116,6 +116,33 @@ def test_batched_green_setup_equals_direct(native, tables):
             assert np.array_equal(a, b)
 
 
+def test_batched_create_failure_releases_operators(native, tables):
+    """A batched creation that fails after queuing operators (an unknown
+    method on its third request) raises, builds none of them and leaves
+    their keys free: the same operators are then built and give the matvec
+    of a one-by-one build."""
+    _log("[gpu] batched creation failure")
+    good = [_request(native, (12, 10, 6), seed=21, lam=433.0), _request(native, (16, 16, 16), seed=22, lam=517.0)]
+    bad = _request(native, (8, 8, 8), seed=23, lam=611.0)
+    native.trim_caches()
+    descs = [r.desc() for r in good] + [bad.desc()]
+    descs[2].method = 7
+    arr = (native.ProblemDesc * 3)(*descs)
+    hs = (native.ctypes.c_void_p * 3)()
+    with pytest.raises(ValueError):
+        native.check(native.lib().tdda_problems_create(arr, 3, native.PREC["fp64"], None, None, 0, 0.1, hs))
+    again = native.Problem.create_many(good, precision="fp64", tables=tables)
+    assert [p.grid()[2] for p in again] == [False, False]
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal(3 * p.n) + 1j * rng.standard_normal(3 * p.n) for p in again]
+    ya = native.apply(again, xs)
+    del again
+    native.trim_caches()
+    one = [native.Problem(r, precision="fp64", tables=tables) for r in good]
+    for a, b in zip(ya, native.apply(one, xs)):
+        assert np.array_equal(a, b)
+
+
 def test_server_mix_equals_alone_and_oracle(native, tables):
     import dda_oracle as O
     _log("[gpu] server mix vs alone")
