@@ -47,8 +47,10 @@ class Problem {
     // used instead of loading it.  Not for solves with another right-hand
     // side or a starting guess.
     void use_incident_r0();
-    // E = al P 3/(eps + 2) (DDAModel.cpp:1185-1192) into out (device, 3N complex)
-    void field(double2* out, hipStream_t st) const;
+    // E = al P 3/(eps + 2) (DDAModel.cpp:1185-1192) into out (device, 3N
+    // complex); from_y: the solution vector holds Y = al P (the end of a
+    // preconditioned solve whose scaling back was skipped)
+    void field(double2* out, hipStream_t st, bool from_y = false) const;
     static size_t estimate_bytes(const ProblemDesc& d, bool refined, bool need_operator);
 
     ProblemDesc desc;

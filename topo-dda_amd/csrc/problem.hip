@@ -100,14 +100,19 @@ k_problem_setup_round(const RoundJob* __restrict__ jobs, int njobs, const char* 
     for (int k = 0; k < 3; ++k) J.sol[3 * i + k] = make_double2(0.0, 0.0);
 }
 
+// ial64 non-null: Pv holds Y = al P (a preconditioned solve), P = Y alpha
 __global__ void k_problem_field(SetupParams P, const double* __restrict__ rho, const double2* __restrict__ al64,
-                                const double2* __restrict__ Pv, double2* __restrict__ out) {
+                                const double2* __restrict__ ial64, const double2* __restrict__ Pv,
+                                double2* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const double2 eps = eps_of(P, rho[i]);
     const double2 lf = make_double2((eps.x + 2.0) / 3.0, eps.y / 3.0);
     const double2 f = ddiv(al64[i], lf);
-    for (int k = 0; k < 3; ++k) out[3 * i + k] = cmul(f, Pv[3 * i + k]);
+    for (int k = 0; k < 3; ++k) {
+        const double2 p = ial64 != nullptr ? dcmul(Pv[3 * i + k], ial64[i]) : Pv[3 * i + k];
+        out[3 * i + k] = cmul(f, p);
+    }
 }
 
 SetupParams params_of(const Problem& p) {
@@ -237,10 +242,10 @@ void Problem::use_incident_r0() {
     }
 }
 
-void Problem::field(double2* out, hipStream_t st) const {
+void Problem::field(double2* out, hipStream_t st, bool from_y) const {
     SetupParams P = params_of(*this);
     hipLaunchKernelGGL(k_problem_field, dim3((ndip + 255) / 256), dim3(256), 0, st, P, rho.get(), sys->al64.get(),
-                       sys->P.get(), out);
+                       from_y ? sys->ial64.get() : nullptr, sys->P.get(), out);
     TDDA_HIP(hipGetLastError());
 }
 

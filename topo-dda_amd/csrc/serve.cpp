@@ -295,12 +295,12 @@ void Server::retire(const std::vector<int>& fin, const KScalars* snap) {
         rt.shape = std::make_tuple(sl.req.desc.nx, sl.req.desc.ny, sl.req.desc.nz);
         rt.prob = std::move(sl.prob);
         // a preconditioned solve ends with Y = al P in P (the batch is gated
-        // off for it, so later chunks leave it alone)
-        if (rt.prob->sys->jacobi()) rt.prob->sys->scale_solution(false, main_);
+        // off for it, so later chunks leave it alone): the field is formed
+        // from Y, and without kept fields nothing reads P again
         if (opt_.keep_fields) {
             rt.n = 3ull * rt.prob->ndip;
             rt.dev.alloc(rt.n);
-            rt.prob->field(rt.dev.get(), main_);
+            rt.prob->field(rt.dev.get(), main_, rt.prob->sys->jacobi());
             auto& pool = pinned_fields_[rt.n];
             if (!pool.empty()) {
                 rt.host = pool.back();
