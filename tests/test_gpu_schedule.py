@@ -54,9 +54,12 @@ def test_tuner_decisions_and_results():
                     assert d["ms"] > 0
                     has_alt = (p in ("xfwd", "xinv", "y") and L in (80, 96, 128)) or (p == "z" and L in (40, 48, 64))
                     assert (d["ms_alt"] is not None) == has_alt, (prec, p, L)
-                    # variant 1 only when at least 3 % faster (ties at the printed precision: either)
-                    if not (has_alt and abs(d["ms_alt"] - 0.97 * d["ms"]) < 2e-7):
-                        assert d["chosen"] == (1 if has_alt and d["ms_alt"] < 0.97 * d["ms"] else 0)
+                    # variant 1 only when at least 3 % faster, 15 % for a Z split of
+                    # another kernel class (all Z alternatives here) (ties at the
+                    # printed precision: either)
+                    m = 0.85 if p == "z" else 0.97
+                    if not (has_alt and abs(d["ms_alt"] - m * d["ms"]) < 2e-7):
+                        assert d["chosen"] == (1 if has_alt and d["ms_alt"] < m * d["ms"] else 0)
         # (16,16,16) -> (32, 32), (48,48,24) -> (96, 48); (64,40,32) has Ly = 80: no fused class
         assert set(fused) == {(32, 32), (96, 48)}
         # plane lengths: 32 and 96 (64x40 has Lx != Ly)
