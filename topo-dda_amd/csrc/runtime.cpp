@@ -445,13 +445,16 @@ void OperatorCache::clear() {
     lru_.clear();
 }
 
+// Builds finish in the order they were enqueued (mostly on one stream), so
+// the scan stops at the first build still running: a cache lookup then costs
+// one event query, not one per operator of the last admission rounds.
 void OperatorCache::reap_locked() {
-    size_t k = 0;
-    for (size_t i = 0; i < unreaped_.size(); ++i) {
+    size_t i = 0;
+    for (; i < unreaped_.size(); ++i) {
         std::shared_ptr<Operator> op = unreaped_[i].lock();
-        if (op && !op->reap()) unreaped_[k++] = unreaped_[i];
+        if (op && !op->reap()) break;
     }
-    unreaped_.resize(k);
+    unreaped_.erase(unreaped_.begin(), unreaped_.begin() + i);
 }
 
 void OperatorCache::touch_locked(const OperatorKey& key) {
